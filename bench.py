@@ -1,0 +1,128 @@
+"""Benchmark: FP64 grid-point steps per second of the 2D incompressible
+pseudo-spectral solver, RK4, 4096^2 (nu = 1e-4, dt = 2e-4, random-phase
+vorticity with E(k) ~ k^4 exp(-(k/4)^2), unit kinetic energy).
+
+One "step" = one full RK4 step (4 right-hand sides, each: inverse y pass with
+the spectral multiplies, x pass with c2r / products / r2c, forward y pass with
+dealias + viscous term + the fused stage update) on one GPU, state resident in
+HBM.  value = Nx*Ny*steps / seconds.
+
+Prints one JSON line.  Also reports the roofline of the dominant kernel
+(algorithmic bytes / average HIP-event duration, against 8 TB/s HBM) and a CPU
+baseline: the reference's own derive2csh + FFTW on one host core (it is serial
+by construction), one RK4 step at the same size, timed in a child process.
+"""
+import argparse
+import json
+import os
+import resource
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "grid-pt steps/s FP64 (2D 4096², 3D 256³ MHD); FNO sim-time/s at ≤5% L2 err"
+
+
+def cpu_baseline(n):
+    def unlimited_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n", str(n), "--steps", "1"],
+                       cwd=ROOT, capture_output=True, text=True, preexec_fn=unlimited_stack, env=env,
+                       timeout=600)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr[-2000:])
+        return None
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": d["value"], "unit": "grid-pt steps/s", "cores": d["cores"], "kind": d["kind"],
+            "sample": f"1 RK4 step (4 derive2csh calls + stage updates) at {n}^2, "
+                      f"{d['seconds']:.1f} s on 1 host core"
+                      + (", TARA Fortran + FFTW 3.3.9 built from the reference sources" if d["kind"] == "reference"
+                         else ", C restatement (not FFTW)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+    n = a.n
+    nu, dt = 1e-4, 2e-4
+
+    # CPU baseline first, in a child process, before this process touches the GPU.
+    cpu = None if a.no_cpu_baseline else cpu_baseline(n)
+
+    import tara_amd as T
+    from tara_amd.ic import random_phase_vorticity
+
+    w0 = random_phase_vorticity(n, seed=1, k0=4.0)
+    s = T.CshSolver(n, n, nu, dt, T.SCHEME_RK4)
+    s.upload(w0)
+    s.advance(a.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.advance(a.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+
+    # Same region again with HIP events around every launch (per kernel class).
+    s.set_timing(True)
+    s.advance(a.steps)
+    s.sync()
+    counts, ms = s.kernel_stats(3)
+    s.set_timing(False)
+    w = s.download()
+    s.close()
+    finite = bool(abs(w).max() < 1e300)
+
+    S = 16.0 * (n // 2 + 1) * n                      # one half spectrum, bytes
+    model = {0: 5 * S, 1: 5 * S, 2: 5 * S}           # algorithmic bytes per launch
+    names = {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd"}
+    avg = {c: ms[c] / counts[c] for c in range(3) if counts[c]}
+    dom = max(avg, key=lambda c: avg[c] * counts[c])
+    achieved = model[dom] / (avg[dom] * 1e-3) / 1e9
+    kernels = {names[c]: {"avg_ms": round(avg[c], 4), "launches": counts[c],
+                          "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}
+    out = {
+        "metric": METRIC,
+        "value": n * n / (ms_per_step * 1e-3),
+        "unit": "grid-pt steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D incompressible hydro, vorticity pseudo-spectral, RK4, {n}^2, FP64",
+                   "nx": n, "ny": n, "nu": nu, "dt": dt, "scheme": "rk4", "dealias": "circular 2/3",
+                   "ic": "random-phase E(k)~k^4 exp(-(k/4)^2), unit KE, seed 1"},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": model[dom]},
+        "kernels": kernels,
+        "step_model_bytes": 60 * S,
+        "step_model_GBps": round(60 * S / (ms_per_step * 1e-3) / 1e9, 1),
+        "state_finite": finite,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

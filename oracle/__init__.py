@@ -1,0 +1,254 @@
+"""TEST INFRASTRUCTURE ONLY -- the CPU oracle.
+
+ctypes bindings for
+  * ``liboracle.so``        : the plain-C restatement (tara_oracle.c) of the
+                              reference arithmetic, and
+  * ``_ref/libtara_ref.so`` : the reference's own Fortran routines linked with
+                              the vendored FFTW 3.3.9, built from the reference
+                              sources by ``oracle/Makefile`` (``make ref``).
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
+import this package, and only as the checker / CPU baseline.  The product path
+(``tara_amd``) never imports it.
+
+Arrays follow the reference layout: Fortran column-major complex128 half
+spectra of shape (Nh, Ny[, Nz]) with Nh = Nx//2 + 1 (numpy ``order='F'``), real
+fields (Nx, Ny[, Nz]) likewise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+_orc = None
+_ref = None
+
+
+def _ptr(a):
+    assert a.flags.f_contiguous or a.ndim == 1, "arrays must be Fortran-ordered"
+    return a.ctypes.data_as(_dp)
+
+
+def lib():
+    """Load liboracle.so (the C restatement)."""
+    global _orc
+    if _orc is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        _orc = C.CDLL(path)
+        _orc.orc_dft.argtypes = [C.c_int, _dp, C.c_long, C.c_int]
+        for n in ("orc_r2c_2d", "orc_c2r_2d"):
+            getattr(_orc, n).argtypes = [C.c_int, C.c_int, _dp, _dp]
+        for n in ("orc_r2c_3d", "orc_c2r_3d"):
+            getattr(_orc, n).argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
+        _orc.orc_derive2csh.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp]
+        _orc.orc_ab2csh.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        _orc.orc_ab2csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        _orc.orc_rk4csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+        _orc.orc_ic2csh.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5 + [_dp, _dp]
+        _orc.orc_derive3com.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _orc.orc_ab3com.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 4
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(os.path.join(_HERE, "_ref", "libtara_ref.so"))
+
+
+def ref():
+    """Load _ref/libtara_ref.so (reference Fortran + vendored FFTW)."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(_HERE, "_ref", "libtara_ref.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref` where the reference is present")
+        _ref = C.CDLL(path)
+        _ref.ref_ic2csh.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5 + [_dp, _dp]
+        _ref.ref_ab2csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        _ref.ref_rk4csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+    return _ref
+
+
+def zeros_spec(nx, ny, nz=None):
+    shape = (nx // 2 + 1, ny) if nz is None else (nx // 2 + 1, ny, nz)
+    return np.zeros(shape, dtype=np.complex128, order="F")
+
+
+# --------------------------------------------------------------- transforms
+def r2c(real):
+    real = np.asfortranarray(real, dtype=np.float64)
+    if real.ndim == 2:
+        nx, ny = real.shape
+        out = zeros_spec(nx, ny)
+        lib().orc_r2c_2d(nx, ny, _ptr(real), _ptr(out))
+    else:
+        nx, ny, nz = real.shape
+        out = zeros_spec(nx, ny, nz)
+        lib().orc_r2c_3d(nx, ny, nz, _ptr(real), _ptr(out))
+    return out
+
+
+def c2r(spec, nx):
+    spec = np.asfortranarray(spec, dtype=np.complex128)
+    if spec.ndim == 2:
+        ny = spec.shape[1]
+        out = np.zeros((nx, ny), order="F")
+        lib().orc_c2r_2d(nx, ny, _ptr(spec), _ptr(out))
+    else:
+        ny, nz = spec.shape[1:]
+        out = np.zeros((nx, ny, nz), order="F")
+        lib().orc_c2r_3d(nx, ny, nz, _ptr(spec), _ptr(out))
+    return out
+
+
+def dft(z, sign):
+    z = np.array(z, dtype=np.complex128)
+    lib().orc_dft(z.size, z.ctypes.data_as(_dp), 1, sign)
+    return z
+
+
+# ----------------------------------------------------------------- 2D hydro
+def derive2csh(w, nx, nu):
+    w = np.asfortranarray(w, dtype=np.complex128)
+    out = np.zeros_like(w, order="F")
+    lib().orc_derive2csh(nx, w.shape[1], nu, _ptr(w), _ptr(out))
+    return out
+
+
+def ab2csh(w, f_old, f_new, nx, dt):
+    w, f_old, f_new = (np.asfortranarray(a, dtype=np.complex128) for a in (w, f_old, f_new))
+    out = np.zeros_like(w, order="F")
+    lib().orc_ab2csh(nx, w.shape[1], dt, _ptr(w), _ptr(out), _ptr(f_old), _ptr(f_new))
+    return out
+
+
+def ab2csh_steps(w, hist, nx, nu, dt, nsteps):
+    w = np.array(w, dtype=np.complex128, order="F")
+    hist = np.array(hist, dtype=np.complex128, order="F")
+    lib().orc_ab2csh_steps(nx, w.shape[1], nu, dt, nsteps, _ptr(w), _ptr(hist))
+    return w, hist
+
+
+def rk4csh_steps(w, nx, nu, dt, nsteps):
+    w = np.array(w, dtype=np.complex128, order="F")
+    lib().orc_rk4csh_steps(nx, w.shape[1], nu, dt, nsteps, _ptr(w))
+    return w
+
+
+def ic2csh(nx, ny, lx_in=2.0, ly_in=2.0, w0=25.0, m0=2.0, d_in=3.0):
+    omega = np.zeros((nx, ny), order="F")
+    wk = zeros_spec(nx, ny)
+    lib().orc_ic2csh(nx, ny, lx_in, ly_in, w0, m0, d_in, _ptr(omega), _ptr(wk))
+    return omega, wk
+
+
+# ------------------------------------------------------------------ 3D MHD
+class MhdParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("lx", "ly", "lz", "mu", "eta", "cs", "dt")]
+
+
+def _ptr_array(arrs):
+    return (C.c_void_p * 8)(*[a.ctypes.data for a in arrs])
+
+
+def derive3com(state, nx, lx, ly, lz, mu, eta, cs=10.0):
+    """state: list of 8 (Nh,Ny,Nz) spectra (rho, rho_ux, rho_uy, rho_uz, E, Bx, By, Bz)."""
+    state = [np.asfortranarray(s, dtype=np.complex128) for s in state]
+    ny, nz = state[0].shape[1:]
+    out = [np.zeros_like(state[0], order="F") for _ in range(8)]
+    p = MhdParams(lx, ly, lz, mu, eta, cs, 0.0)
+    lib().orc_derive3com(nx, ny, nz, C.byref(p), _ptr_array(state), _ptr_array(out))
+    return out
+
+
+def ab3com(state, f_old, f_new, nx, dt):
+    state = [np.asfortranarray(s, dtype=np.complex128) for s in state]
+    f_old = [np.asfortranarray(s, dtype=np.complex128) for s in f_old]
+    f_new = [np.asfortranarray(s, dtype=np.complex128) for s in f_new]
+    ny, nz = state[0].shape[1:]
+    out = [np.zeros_like(state[0], order="F") for _ in range(8)]
+    lib().orc_ab3com(nx, ny, nz, dt, _ptr_array(state), _ptr_array(out), _ptr_array(f_old), _ptr_array(f_new))
+    return out
+
+
+# ------------------------------------------------- reference (FFTW) routines
+def _i(v):
+    return C.byref(C.c_int(v))
+
+
+def _d(v):
+    return C.byref(C.c_double(v))
+
+
+def ref_derive2csh(w, nx, nu):
+    w = np.array(w, dtype=np.complex128, order="F")
+    nh, ny = w.shape
+    dum = np.zeros_like(w, order="F")
+    out = np.zeros_like(w, order="F")
+    ref().derive2csh_(_i(nx), _i(ny), _i(nh), _d(nu), _d(0.0), _ptr(w), _ptr(dum), _ptr(dum), _ptr(out))
+    return out
+
+
+def ref_ab2csh(w, f_old, f_new, nx, dt):
+    w, f_old, f_new = (np.array(a, dtype=np.complex128, order="F") for a in (w, f_old, f_new))
+    nh, ny = w.shape
+    out = np.zeros_like(w, order="F")
+    ref().ab2csh_(_i(nx), _i(ny), _i(nh), _d(0.0), _d(0.0), _d(dt), _ptr(w), _ptr(out), _ptr(f_old), _ptr(f_new))
+    return out
+
+
+def ref_ic2csh(nx, ny, lx_in=2.0, ly_in=2.0, w0=25.0, m0=2.0, d_in=3.0):
+    omega = np.zeros((nx, ny), order="F")
+    wk = zeros_spec(nx, ny)
+    ref().ref_ic2csh(nx, ny, lx_in, ly_in, w0, m0, d_in, _ptr(omega), _ptr(wk))
+    return omega, wk
+
+
+def ref_ab2csh_steps(w, hist, nx, nu, dt, nsteps):
+    w = np.array(w, dtype=np.complex128, order="F")
+    hist = np.array(hist, dtype=np.complex128, order="F")
+    ref().ref_ab2csh_steps(nx, w.shape[1], nu, dt, nsteps, _ptr(w), _ptr(hist))
+    return w, hist
+
+
+def ref_rk4csh_steps(w, nx, nu, dt, nsteps):
+    w = np.array(w, dtype=np.complex128, order="F")
+    ref().ref_rk4csh_steps(nx, w.shape[1], nu, dt, nsteps, _ptr(w))
+    return w
+
+
+class _Comm(C.Structure):
+    """COMMON /comm/ of src/main3com.f95:56 / src/derive3com.f95:67."""
+    _pack_ = 4
+    _fields_ = [("iret", C.c_int), ("thread_num", C.c_int)] + [
+        (n, C.c_double) for n in ("time_max", "Lx", "Ly", "Lz", "dx", "dy", "dz", "spheat",
+                                  "mu", "ms", "CS", "mu_0", "eta", "dt", "kf")]
+
+
+def ref_comm():
+    return _Comm.in_dll(ref(), "comm_")
+
+
+def ref_derive3com(state, nx, lx, ly, lz, cs=10.0, threads=1):
+    """Calls the reference derive3com (diffusion hard-wired to 1/450, Q9)."""
+    state = [np.array(s, dtype=np.complex128, order="F") for s in state]
+    nh, ny, nz = state[0].shape
+    cm = ref_comm()
+    cm.thread_num = threads
+    cm.Lx, cm.Ly, cm.Lz = lx, ly, lz
+    cm.CS = cs
+    cm.kf = 1.0
+    cm.dx, cm.dy, cm.dz = lx / nx, ly / ny, lz / nz
+    old = [np.zeros_like(state[0], order="F") for _ in range(8)]
+    new = [np.zeros_like(state[0], order="F") for _ in range(8)]
+    args = [_i(nx), _i(ny), _i(nz), _i(nh), _d(3.14159265358979323846), _d(0.0)]
+    args += [_ptr(s) for s in state]
+    for o, n in zip(old, new):
+        args += [_ptr(o), _ptr(n)]
+    ref().derive3com_(*args)
+    return new

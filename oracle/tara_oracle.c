@@ -1,0 +1,436 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the CPU oracle.  Plain-C restatement of the
+ * reference arithmetic on the hot path, used only as the checker by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The product never
+ * links or calls it.
+ *
+ * Each routine cites the reference file:line it restates.  Arithmetic is IEEE
+ * double with FMA contraction disabled at compile time (Makefile), evaluated
+ * in the reference's expression order where that order is visible in the
+ * Fortran source.  The transforms restate FFTW 3.3.9's definitions (unnormalised,
+ * e^{-2 pi i jk/n} forward) and its multi-dimensional ordering
+ * (lib/fftw/rdft/rank-geq2-rdft2.c:40-68): r2c = real pass along the halved
+ * (Fortran-first) axis then complex passes along the others; c2r = complex
+ * passes first, then the 1-D c2r along x, which reads only the real part of
+ * the kx = 0 and kx = Nx/2 entries.
+ */
+#include "tara_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define RE(p, i) ((p)[2 * (size_t)(i)])
+#define IM(p, i) ((p)[2 * (size_t)(i) + 1])
+
+/* ------------------------------------------------------------------------ */
+/* 1-D complex DFT: radix-2 iterative for powers of two, direct O(n^2) with   */
+/* long-double accumulation otherwise.  Twiddles from long-double cosl/sinl. */
+/* ------------------------------------------------------------------------ */
+
+static void twiddle(int n, long k, int sign, double *c, double *s)
+{
+    long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)(k % n) / (long double)n;
+    *c = (double)cosl(a);
+    *s = (double)(sign * sinl(a));
+}
+
+void orc_dft(int n, double *z, long stride, int sign)
+{
+    if (n <= 1) return;
+    double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    for (int j = 0; j < n; ++j) {
+        buf[2 * j] = z[2 * (size_t)j * stride];
+        buf[2 * j + 1] = z[2 * (size_t)j * stride + 1];
+    }
+    if ((n & (n - 1)) == 0) {
+        /* bit reversal */
+        for (int i = 1, j = 0; i < n; ++i) {
+            int bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) {
+                double tr = buf[2 * i], ti = buf[2 * i + 1];
+                buf[2 * i] = buf[2 * j]; buf[2 * i + 1] = buf[2 * j + 1];
+                buf[2 * j] = tr; buf[2 * j + 1] = ti;
+            }
+        }
+        double *tw = (double *)malloc(sizeof(double) * (size_t)n);   /* n/2 complex twiddles */
+        for (int k = 0; k < n / 2; ++k) twiddle(n, k, sign, &tw[2 * k], &tw[2 * k + 1]);
+        for (int len = 2; len <= n; len <<= 1) {
+            int step = n / len;
+            for (int i = 0; i < n; i += len) {
+                for (int k = 0; k < len / 2; ++k) {
+                    double wr = tw[2 * k * step], wi = tw[2 * k * step + 1];
+                    double *a = &buf[2 * (i + k)], *b = &buf[2 * (i + k + len / 2)];
+                    double br = b[0] * wr - b[1] * wi;
+                    double bi = b[0] * wi + b[1] * wr;
+                    b[0] = a[0] - br; b[1] = a[1] - bi;
+                    a[0] = a[0] + br; a[1] = a[1] + bi;
+                }
+            }
+        }
+        free(tw);
+        for (int j = 0; j < n; ++j) {
+            z[2 * (size_t)j * stride] = buf[2 * j];
+            z[2 * (size_t)j * stride + 1] = buf[2 * j + 1];
+        }
+    } else {
+        for (int k = 0; k < n; ++k) {
+            long double sr = 0.0L, si = 0.0L;
+            for (int j = 0; j < n; ++j) {
+                long double a = 2.0L * 3.141592653589793238462643383279502884L *
+                                (long double)(((long)j * k) % n) / (long double)n;
+                long double c = cosl(a), s = sign * sinl(a);
+                sr += buf[2 * j] * c - buf[2 * j + 1] * s;
+                si += buf[2 * j] * s + buf[2 * j + 1] * c;
+            }
+            z[2 * (size_t)k * stride] = (double)sr;
+            z[2 * (size_t)k * stride + 1] = (double)si;
+        }
+    }
+    free(buf);
+}
+
+/* 1-D r2c along a contiguous real row of length n -> n/2+1 complex. */
+static void r2c_row(int n, const double *x, double *out)
+{
+    double *z = (double *)calloc(2 * (size_t)n, sizeof(double));
+    for (int j = 0; j < n; ++j) z[2 * j] = x[j];
+    orc_dft(n, z, 1, -1);
+    memcpy(out, z, sizeof(double) * 2 * (size_t)(n / 2 + 1));
+    free(z);
+}
+
+/* 1-D c2r: n/2+1 complex -> n real.  Imaginary parts of DC and (even n)
+   Nyquist are not read, as FFTW's halfcomplex c2r codelets. */
+static void c2r_row(int n, const double *in, double *x)
+{
+    double *z = (double *)calloc(2 * (size_t)n, sizeof(double));
+    int nh = n / 2 + 1;
+    z[0] = in[0];
+    z[1] = 0.0;
+    for (int k = 1; k < nh; ++k) {
+        if (2 * k == n) { z[2 * k] = in[2 * k]; z[2 * k + 1] = 0.0; continue; }
+        z[2 * k] = in[2 * k];
+        z[2 * k + 1] = in[2 * k + 1];
+        z[2 * (n - k)] = in[2 * k];
+        z[2 * (n - k) + 1] = -in[2 * k + 1];
+    }
+    orc_dft(n, z, 1, +1);
+    for (int j = 0; j < n; ++j) x[j] = z[2 * j];
+    free(z);
+}
+
+void orc_r2c_2d(int nx, int ny, const double *in, double *out)
+{
+    int nh = nx / 2 + 1;
+    for (int j = 0; j < ny; ++j) r2c_row(nx, in + (size_t)j * nx, out + 2 * (size_t)j * nh);
+    for (int i = 0; i < nh; ++i) orc_dft(ny, out + 2 * (size_t)i, nh, -1);
+}
+
+void orc_c2r_2d(int nx, int ny, const double *in, double *out)
+{
+    int nh = nx / 2 + 1;
+    double *t = (double *)malloc(sizeof(double) * 2 * (size_t)nh * ny);
+    memcpy(t, in, sizeof(double) * 2 * (size_t)nh * ny);
+    for (int i = 0; i < nh; ++i) orc_dft(ny, t + 2 * (size_t)i, nh, +1);
+    for (int j = 0; j < ny; ++j) c2r_row(nx, t + 2 * (size_t)j * nh, out + (size_t)j * nx);
+    free(t);
+}
+
+void orc_r2c_3d(int nx, int ny, int nz, const double *in, double *out)
+{
+    int nh = nx / 2 + 1;
+    size_t plane = (size_t)nh * ny;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            r2c_row(nx, in + ((size_t)k * ny + j) * nx, out + 2 * (k * plane + (size_t)j * nh));
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nh; ++i) orc_dft(ny, out + 2 * (k * plane + i), nh, -1);
+    for (size_t p = 0; p < plane; ++p) orc_dft(nz, out + 2 * p, (long)plane, -1);
+}
+
+void orc_c2r_3d(int nx, int ny, int nz, const double *in, double *out)
+{
+    int nh = nx / 2 + 1;
+    size_t plane = (size_t)nh * ny;
+    double *t = (double *)malloc(sizeof(double) * 2 * plane * nz);
+    memcpy(t, in, sizeof(double) * 2 * plane * nz);
+    for (size_t p = 0; p < plane; ++p) orc_dft(nz, t + 2 * p, (long)plane, +1);
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nh; ++i) orc_dft(ny, t + 2 * (k * plane + i), nh, +1);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            c2r_row(nx, t + 2 * (k * plane + (size_t)j * nh), out + ((size_t)k * ny + j) * nx);
+    free(t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2D incompressible hydro: src/derive2csh.f95, src/ab2csh.f95              */
+/* ------------------------------------------------------------------------ */
+
+/* Wavenumbers of derive2csh: pi is the default-real literal 3.1428
+   (src/derive2csh.f95:16), Lx = Ly = 2 pi (:20-21), kx = 2 pi (i-1)/Lx (:30),
+   ky = 2 pi (j-1)/Ly for j <= Ny/2 and 2 pi ((j-1)-Ny)/Ly above (:32, :48). */
+static double csh_k(int idx0, int n, int halved)
+{
+    const double pi = (double)3.1428f;
+    const double L = 2.0 * pi;
+    double m = (halved || idx0 < n / 2) ? (double)idx0 : ((double)idx0 - (double)n);
+    return 2.0 * pi * m / L;
+}
+
+void orc_derive2csh(int nx, int ny, double nu, const double *w, double *dt_new)
+{
+    int nh = nx / 2 + 1;
+    size_t ns = (size_t)nh * ny, nr = (size_t)nx * ny;
+    double *ukx = calloc(2 * ns, 8), *uky = calloc(2 * ns, 8), *ikxw = calloc(2 * ns, 8), *ikyw = calloc(2 * ns, 8);
+    double *ux = malloc(8 * nr), *uy = malloc(8 * nr), *wx = malloc(8 * nr), *wy = malloc(8 * nr);
+    double *nlkx = malloc(16 * ns), *nlky = malloc(16 * ns);
+    /* psi, u spectra (src/derive2csh.f95:29-57) and i k omega (:59-71) */
+    for (int j = 0; j < ny; ++j) {
+        double ky = csh_k(j, ny, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = csh_k(i, nx, 1);
+            size_t p = (size_t)j * nh + i;
+            double wr = RE(w, p), wi = IM(w, p);
+            double pr, pim;
+            if (i == 0 && j == 0) { pr = 0.0; pim = 0.0; }
+            else { double k2 = kx * kx + ky * ky; pr = wr / k2; pim = wi / k2; }
+            if (i == 0 && j == 0) { RE(ukx, p) = 0; IM(ukx, p) = 0; RE(uky, p) = 0; IM(uky, p) = 0; }
+            else {
+                RE(ukx, p) = -(ky * pim); IM(ukx, p) = ky * pr;       /* +i ky psi */
+                RE(uky, p) = kx * pim;    IM(uky, p) = -(kx * pr);    /* -i kx psi */
+            }
+            RE(ikxw, p) = -(kx * wi); IM(ikxw, p) = kx * wr;
+            RE(ikyw, p) = -(ky * wi); IM(ikyw, p) = ky * wr;
+        }
+    }
+    /* four c2r (:73-87), normalise and products (:89-98) */
+    orc_c2r_2d(nx, ny, ukx, ux);
+    orc_c2r_2d(nx, ny, uky, uy);
+    orc_c2r_2d(nx, ny, ikxw, wx);
+    orc_c2r_2d(nx, ny, ikyw, wy);
+    double norm = (double)nx * (double)ny;
+    for (size_t q = 0; q < nr; ++q) {
+        double a = ux[q] / norm, b = uy[q] / norm, c = wx[q] / norm, d = wy[q] / norm;
+        wx[q] = a * c;
+        wy[q] = b * d;
+    }
+    /* two r2c (:100-106) */
+    orc_r2c_2d(nx, ny, wx, nlkx);
+    orc_r2c_2d(nx, ny, wy, nlky);
+    /* circular dealias (:108-125), viscous term (:127-139), negate (:141-145) */
+    double cut = ((double)(nx + ny) / 2.0) / 3.0;
+    for (int j = 0; j < ny; ++j) {
+        double ky = csh_k(j, ny, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = csh_k(i, nx, 1);
+            size_t p = (size_t)j * nh + i;
+            double ar = RE(nlkx, p), ai = IM(nlkx, p), br = RE(nlky, p), bi = IM(nlky, p);
+            if (sqrt(kx * kx + ky * ky) >= cut) { ar = ai = br = bi = 0.0; }
+            double vr = nu * (kx * kx + ky * ky);
+            double nr_ = (ar + br) + vr * RE(w, p);
+            double ni_ = (ai + bi) + vr * IM(w, p);
+            RE(dt_new, p) = -nr_;
+            IM(dt_new, p) = -ni_;
+        }
+    }
+    free(ukx); free(uky); free(ikxw); free(ikyw); free(ux); free(uy); free(wx); free(wy);
+    free(nlkx); free(nlky);
+}
+
+/* src/ab2csh.f95:10-14: new = w + (3/2 f_n - 1/2 f_{n-1}) dt */
+void orc_ab2csh(int nx, int ny, double dt, const double *w, double *wn,
+                const double *f_old, const double *f_new)
+{
+    size_t n2 = 2 * (size_t)(nx / 2 + 1) * ny;
+    for (size_t q = 0; q < n2; ++q) wn[q] = w[q] + (1.5 * f_new[q] - 0.5 * f_old[q]) * dt;
+}
+
+/* Loop of src/main2csh.f95:114-126 (derive, ab2, rotate).  hist is the
+   f_{n-1} array; the caller chooses its initial contents (zero by contract). */
+void orc_ab2csh_steps(int nx, int ny, double nu, double dt, int nsteps, double *w, double *hist)
+{
+    size_t ns = (size_t)(nx / 2 + 1) * ny;
+    double *fn = malloc(16 * ns), *wn = malloc(16 * ns);
+    for (int t = 0; t < nsteps; ++t) {
+        orc_derive2csh(nx, ny, nu, w, fn);
+        orc_ab2csh(nx, ny, dt, w, wn, hist, fn);
+        memcpy(hist, fn, 16 * ns);
+        memcpy(w, wn, 16 * ns);
+    }
+    free(fn); free(wn);
+}
+
+/* Classic RK4 over derive2csh: the intended scheme of src/rk42css.f95:15-48
+   (k1..k4 at stage offsets dt/2, dt/2, dt; w += dt/6 (k1 + 2k2 + 2k3 + k4)). */
+void orc_rk4csh_steps(int nx, int ny, double nu, double dt, int nsteps, double *w)
+{
+    size_t n2 = 2 * (size_t)(nx / 2 + 1) * ny;
+    double *k1 = malloc(8 * n2), *k2 = malloc(8 * n2), *k3 = malloc(8 * n2), *k4 = malloc(8 * n2), *s = malloc(8 * n2);
+    for (int t = 0; t < nsteps; ++t) {
+        orc_derive2csh(nx, ny, nu, w, k1);
+        for (size_t q = 0; q < n2; ++q) s[q] = w[q] + (0.5 * dt) * k1[q];
+        orc_derive2csh(nx, ny, nu, s, k2);
+        for (size_t q = 0; q < n2; ++q) s[q] = w[q] + (0.5 * dt) * k2[q];
+        orc_derive2csh(nx, ny, nu, s, k3);
+        for (size_t q = 0; q < n2; ++q) s[q] = w[q] + dt * k3[q];
+        orc_derive2csh(nx, ny, nu, s, k4);
+        for (size_t q = 0; q < n2; ++q)
+            w[q] = w[q] + (dt / 6.0) * (((k1[q] + 2.0 * k2[q]) + 2.0 * k3[q]) + k4[q]);
+    }
+    free(k1); free(k2); free(k3); free(k4); free(s);
+}
+
+/* Shear-layer IC of src/main2csh.f95:84-110 (x-fastest real field, then r2c). */
+void orc_ic2csh(int nx, int ny, double lx_in, double ly_in, double w0, double m0,
+                double d_in, double *omega, double *omegak)
+{
+    const double pi = 3.14159265358979323846;
+    double lx = lx_in * pi, ly = ly_in * pi;
+    double dx = lx / (double)nx, dy = ly / (double)ny, d = d_in * pi;
+    for (int j = 0; j < ny; ++j) {
+        double y = (double)j * dy;
+        for (int i = 0; i < nx; ++i) {
+            double x = (double)i * dx;
+            double c1 = cosh((y + 0.5 * pi) / d), c2 = cosh((y - 0.5 * pi) / d);
+            double c3 = cosh((y + 1.5 * pi) / d), c4 = cosh((y - 1.5 * pi) / d);
+            double v = w0 / (c1 * c1) - w0 / (c2 * c2);
+            v = v - w0 / (c3 * c3) + w0 / (c4 * c4);
+            v = v + 0.10 * cos(m0 * x);
+            omega[(size_t)j * nx + i] = v;
+        }
+    }
+    orc_r2c_2d(nx, ny, omega, omegak);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3D compressible isothermal MHD: src/derive3com.f95, src/ab3com.f95       */
+/* Only the live dataflow is restated: E is frozen by ab3com (ab3com.f95:51) */
+/* and P = CS^2 rho (derive3com.f95:395), so E, curl B, the energy fluxes    */
+/* and the (commented-out) forcing never reach rho, rho u or B.             */
+/* ------------------------------------------------------------------------ */
+
+/* k = 2 pi (i-1)/L with true pi (derive3com.f95:245-247, 260-262); the
+   second half of y and z uses (i-1)-N.  Returned in the reference's
+   evaluation order ((2 pi) m) / L. */
+static double com_k(int idx0, int n, int halved, double L)
+{
+    const double pi = 3.14159265358979323846;
+    double m = (halved || idx0 < n / 2) ? (double)idx0 : (double)(idx0 - n);
+    return 2.0 * pi * m / L;
+}
+
+void orc_derive3com(int nx, int ny, int nz, const orc_mhd_params *p,
+                    const double *const st[8], double *const rhs[8])
+{
+    int nh = nx / 2 + 1;
+    size_t ns = (size_t)nh * ny * nz, nr = (size_t)nx * ny * nz;
+    double norm = (double)nx * (double)ny * (double)nz;
+    /* c2r of rho, rho u, B (derive3com.f95:94-140); E is skipped (dead) */
+    static const int live[7] = {0, 1, 2, 3, 5, 6, 7};
+    double *r[8] = {0};
+    for (int f = 0; f < 7; ++f) {
+        r[live[f]] = malloc(8 * nr);
+        orc_c2r_3d(nx, ny, nz, st[live[f]], r[live[f]]);
+        for (size_t q = 0; q < nr; ++q) r[live[f]][q] = r[live[f]][q] / norm;
+    }
+    double *rho = r[0], *rux = r[1], *ruy = r[2], *ruz = r[3], *bx = r[5], *by = r[6], *bz = r[7];
+    /* velocity, B^2 (:176-186), pressure and fluxes (:395-436) */
+    enum { UX, UY, UZ, MX1, MX2, MX3, MY2, MY3, MZ1, MZ2, MZ3, MG1, MG2, MG3, NF };
+    double *fl[NF];
+    for (int f = 0; f < NF; ++f) fl[f] = malloc(8 * nr);
+    double cs2 = p->cs * p->cs;
+    for (size_t q = 0; q < nr; ++q) {
+        double ux = rux[q] / rho[q], uy = ruy[q] / rho[q], uz = ruz[q] / rho[q];
+        double b2 = bx[q] * bx[q] + by[q] * by[q] + bz[q] * bz[q];
+        double P = cs2 * rho[q];
+        fl[UX][q] = ux; fl[UY][q] = uy; fl[UZ][q] = uz;
+        fl[MX1][q] = rux[q] * ux + P + b2 / 2.0 - bx[q] * bx[q];
+        fl[MX2][q] = rux[q] * uy - bx[q] * by[q];             /* == Mom_y_1 */
+        fl[MX3][q] = rux[q] * uz - bx[q] * bz[q];
+        fl[MY2][q] = ruy[q] * uy + P + b2 / 2.0 - by[q] * by[q];
+        fl[MY3][q] = ruy[q] * uz - by[q] * bz[q];
+        fl[MZ1][q] = ruz[q] * ux - bz[q] * bx[q];
+        fl[MZ2][q] = ruz[q] * uy - bz[q] * by[q];
+        fl[MZ3][q] = ruz[q] * uz + P + b2 / 2.0 - bz[q] * bz[q];
+        fl[MG1][q] = ux * by[q] - uy * bx[q];                 /* Mag_x_1 == Mag_y_1 */
+        fl[MG2][q] = ux * bz[q] - uz * bx[q];                 /* Mag_x_2 == Mag_z_1 */
+        fl[MG3][q] = uy * bz[q] - uz * by[q];                 /* Mag_y_2 == Mag_z_2 */
+    }
+    double *fk[NF];
+    for (int f = 0; f < NF; ++f) { fk[f] = malloc(16 * ns); orc_r2c_3d(nx, ny, nz, fl[f], fk[f]); free(fl[f]); }
+    for (int f = 0; f < 7; ++f) free(r[live[f]]);
+
+    /* i k . flux, k^2 diffusion, spherical mask (:574-843), assembly (:1018-1037) */
+    double cut = ((double)(nx + ny + nz) / 3.0) / 3.0 + 0;
+    const double *rk[3] = {st[1], st[2], st[3]};
+    for (int k = 0; k < nz; ++k) {
+        double kz = com_k(k, nz, 0, p->lz);
+        for (int j = 0; j < ny; ++j) {
+            double ky = com_k(j, ny, 0, p->ly);
+            for (int i = 0; i < nh; ++i) {
+                double kx = com_k(i, nx, 1, p->lx);
+                size_t q = ((size_t)k * ny + j) * nh + i;
+                int m = sqrt(kx * kx + ky * ky + kz * kz) >= cut;
+                double kk[3] = {kx, ky, kz};
+                /* i k * z = (-k zi, k zr) */
+#define IKR(kv, a) (m ? 0.0 : -((kv) * IM(a, q)))
+#define IKI(kv, a) (m ? 0.0 : ((kv) * RE(a, q)))
+#define K2R(kv, a) (m ? 0.0 : ((kv) * (kv)) * RE(a, q))
+#define K2I(kv, a) (m ? 0.0 : ((kv) * (kv)) * IM(a, q))
+                /* density: -(ikx rux + iky ruy + ikz ruz) on the state spectra */
+                double sr = (IKR(kk[0], rk[0]) + IKR(kk[1], rk[1])) + IKR(kk[2], rk[2]);
+                double si = (IKI(kk[0], rk[0]) + IKI(kk[1], rk[1])) + IKI(kk[2], rk[2]);
+                RE(rhs[0], q) = -sr; IM(rhs[0], q) = -si;
+                /* momentum: -(ik.Mom_row) - (k^2 u)/visc */
+                const double *mom[3][3] = {{fk[MX1], fk[MX2], fk[MX3]},
+                                           {fk[MX2], fk[MY2], fk[MY3]},
+                                           {fk[MZ1], fk[MZ2], fk[MZ3]}};
+                const double *uk[3] = {fk[UX], fk[UY], fk[UZ]};
+                for (int c = 0; c < 3; ++c) {
+                    double ar = (IKR(kx, mom[c][0]) + IKR(ky, mom[c][1])) + IKR(kz, mom[c][2]);
+                    double ai = (IKI(kx, mom[c][0]) + IKI(ky, mom[c][1])) + IKI(kz, mom[c][2]);
+                    double dr = (K2R(kx, uk[c]) + K2R(ky, uk[c])) + K2R(kz, uk[c]);
+                    double di = (K2I(kx, uk[c]) + K2I(ky, uk[c])) + K2I(kz, uk[c]);
+                    RE(rhs[1 + c], q) = -ar - dr * p->mu;
+                    IM(rhs[1 + c], q) = -ai - di * p->mu;
+                }
+                RE(rhs[4], q) = 0.0; IM(rhs[4], q) = 0.0;
+                /* induction (:1030-1037) */
+                const double *bk[3] = {st[5], st[6], st[7]};
+                double d2r[3], d2i[3];
+                for (int c = 0; c < 3; ++c) {
+                    d2r[c] = (K2R(kx, bk[c]) + K2R(ky, bk[c])) + K2R(kz, bk[c]);
+                    d2i[c] = (K2I(kx, bk[c]) + K2I(ky, bk[c])) + K2I(kz, bk[c]);
+                }
+                double xr = IKR(ky, fk[MG1]) + IKR(kz, fk[MG2]);
+                double xi = IKI(ky, fk[MG1]) + IKI(kz, fk[MG2]);
+                RE(rhs[5], q) = xr - d2r[0] * p->eta; IM(rhs[5], q) = xi - d2i[0] * p->eta;
+                double yr = IKR(kx, fk[MG1]) - IKR(kz, fk[MG3]);
+                double yi = IKI(kx, fk[MG1]) - IKI(kz, fk[MG3]);
+                RE(rhs[6], q) = -yr - d2r[1] * p->eta; IM(rhs[6], q) = -yi - d2i[1] * p->eta;
+                double zr = IKR(kx, fk[MG2]) + IKR(ky, fk[MG3]);
+                double zi = IKI(kx, fk[MG2]) + IKI(ky, fk[MG3]);
+                RE(rhs[7], q) = -zr - d2r[2] * p->eta; IM(rhs[7], q) = -zi - d2i[2] * p->eta;
+#undef IKR
+#undef IKI
+#undef K2R
+#undef K2I
+            }
+        }
+    }
+    for (int f = 0; f < NF; ++f) free(fk[f]);
+}
+
+/* src/ab3com.f95:39-59: AB2 for rho, rho u, B; E is copied (frozen, :51). */
+void orc_ab3com(int nx, int ny, int nz, double dt, const double *const st[8],
+                double *const sn[8], const double *const fo[8], const double *const fn[8])
+{
+    size_t n2 = 2 * (size_t)(nx / 2 + 1) * ny * nz;
+    for (int f = 0; f < 8; ++f)
+        for (size_t q = 0; q < n2; ++q)
+            sn[f][q] = (f == 4) ? st[f][q] : st[f][q] + (1.5 * fn[f][q] - 0.5 * fo[f][q]) * dt;
+}

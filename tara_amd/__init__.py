@@ -1,0 +1,166 @@
+"""tara_amd -- MI355X-native (gfx950) pseudo-spectral hot path of TARA.
+
+Python mirror of the reference's hot-path interface, over the C ABI of
+``libtara_hip.so`` (include/tara_hip.h).  Function names, argument meaning and
+array layout follow the Fortran routines they replace:
+
+  derive2csh(Nx, Ny, Nh, nu, omegak)            src/derive2csh.f95:3
+  ab2csh(Nx, Ny, Nh, dt, omegak, f_old, f_new)  src/ab2csh.f95:3
+  fft_r2c / fft_c2r                             dfftw_plan_dft_{r2c,c2r}_{2d,3d} + execute
+
+Spectral arrays are complex128 half spectra (Nh, Ny[, Nz]) in Fortran order
+(numpy ``order='F'``).  Everything computes on the GPU; there is no CPU
+fallback -- a missing or unusable library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtara_hip.so")
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+SCHEME_AB2 = 1
+SCHEME_RK4 = 2
+
+
+class TaraError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libtara_hip.so (built by ``python -m tara_amd.build``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TaraError(f"{LIB_PATH} not built: run `python -m tara_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.tara_last_error.restype = C.c_char_p
+        for n in ("tara_fft_r2c_2d", "tara_fft_c2r_2d"):
+            getattr(L, n).argtypes = [C.c_int, C.c_int, _dp, _dp]
+        for n in ("tara_fft_r2c_3d", "tara_fft_c2r_3d"):
+            getattr(L, n).argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.tara_derive2csh.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp]
+        L.tara_ab2csh.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        L.tara_csh_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
+        L.tara_csh_upload.argtypes = [C.c_void_p, _dp]
+        L.tara_csh_download.argtypes = [C.c_void_p, _dp]
+        L.tara_csh_advance.argtypes = [C.c_void_p, C.c_int]
+        L.tara_csh_sync.argtypes = [C.c_void_p]
+        L.tara_csh_destroy.argtypes = [C.c_void_p]
+        L.tara_csh_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.tara_csh_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TaraError(f"tara error {rc}: {lib().tara_last_error().decode()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _spec(a):
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def _real(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ transforms
+def fft_r2c(real):
+    """r2c of a (Nx, Ny[, Nz]) real field -> (Nh, Ny[, Nz]) half spectrum (FFTW semantics)."""
+    real = _real(real)
+    if real.ndim == 2:
+        nx, ny = real.shape
+        out = np.zeros((nx // 2 + 1, ny), np.complex128, order="F")
+        _check(lib().tara_fft_r2c_2d(nx, ny, _p(real), _p(out)))
+    else:
+        nx, ny, nz = real.shape
+        out = np.zeros((nx // 2 + 1, ny, nz), np.complex128, order="F")
+        _check(lib().tara_fft_r2c_3d(nx, ny, nz, _p(real), _p(out)))
+    return out
+
+
+def fft_c2r(spec, nx):
+    """Unnormalised c2r (FFTW semantics); the input is left intact."""
+    spec = _spec(spec)
+    if spec.ndim == 2:
+        ny = spec.shape[1]
+        out = np.zeros((nx, ny), order="F")
+        _check(lib().tara_fft_c2r_2d(nx, ny, _p(spec), _p(out)))
+    else:
+        ny, nz = spec.shape[1:]
+        out = np.zeros((nx, ny, nz), order="F")
+        _check(lib().tara_fft_c2r_3d(nx, ny, nz, _p(spec), _p(out)))
+    return out
+
+
+# ------------------------------------------------- 2D incompressible (csh)
+def derive2csh(Nx, Ny, Nh, nu, omegak):
+    """dt_omegak_new = derive2csh(...) (src/derive2csh.f95:3)."""
+    w = _spec(omegak)
+    assert w.shape == (Nh, Ny)
+    out = np.zeros_like(w, order="F")
+    _check(lib().tara_derive2csh(Nx, Ny, Nh, nu, _p(w), _p(out)))
+    return out
+
+
+def ab2csh(Nx, Ny, Nh, dt, omegak, dt_omegak_old, dt_omegak_new):
+    """omegak_new = ab2csh(...) (src/ab2csh.f95:3)."""
+    w, fo, fn = _spec(omegak), _spec(dt_omegak_old), _spec(dt_omegak_new)
+    out = np.zeros_like(w, order="F")
+    _check(lib().tara_ab2csh(Nx, Ny, Nh, dt, _p(w), _p(out), _p(fo), _p(fn)))
+    return out
+
+
+class CshSolver:
+    """Device-resident 2D vorticity solver (the main2csh.f95:114-126 loop on the GPU)."""
+
+    def __init__(self, nx, ny, nu, dt, scheme=SCHEME_AB2):
+        self.nx, self.ny, self.nh = nx, ny, nx // 2 + 1
+        h = C.c_void_p()
+        _check(lib().tara_csh_create(nx, ny, nu, dt, scheme, C.byref(h)))
+        self._h = h
+
+    def upload(self, omegak):
+        w = _spec(omegak)
+        assert w.shape == (self.nh, self.ny)
+        _check(lib().tara_csh_upload(self._h, _p(w)))
+
+    def download(self):
+        out = np.zeros((self.nh, self.ny), np.complex128, order="F")
+        _check(lib().tara_csh_download(self._h, _p(out)))
+        return out
+
+    def advance(self, nsteps):
+        _check(lib().tara_csh_advance(self._h, nsteps))
+
+    def sync(self):
+        _check(lib().tara_csh_sync(self._h))
+
+    def set_timing(self, on):
+        _check(lib().tara_csh_set_timing(self._h, 1 if on else 0))
+
+    def kernel_stats(self, nclass=3):
+        counts = (C.c_long * nclass)()
+        ms = (C.c_double * nclass)()
+        _check(lib().tara_csh_kernel_stats(self._h, nclass, counts, ms))
+        return list(counts), list(ms)
+
+    def close(self):
+        if self._h:
+            lib().tara_csh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,84 @@
+"""Builds libtara_hip.so (gfx950) in-tree from tara_amd/csrc/*.hip, and the
+Fortran drop-in wrappers (libtara_fortran.so) with amdflang.
+
+    python -m tara_amd.build            # compile everything
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libtara_hip.so")
+FLIB = os.path.join(HERE, "libtara_fortran.so")
+ARCH = os.environ.get("TARA_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed FP32 instructions are kept out of every code object
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", "-Wno-unknown-warning-option",
+    "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} ... {cmd[-1]}")
+    return r
+
+
+def build_hip(jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    objs = [os.path.join(OBJ, s[:-4] + ".o") for s in srcs]
+
+    def one(pair):
+        src, obj = pair
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+
+    with ThreadPoolExecutor(max_workers=jobs or min(8, os.cpu_count() or 4)) as ex:
+        list(ex.map(one, zip(srcs, objs)))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran():
+    """ISO_C_BINDING wrappers keeping the reference's subroutine names."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    fdir = os.path.join(HERE, "fortran")
+    srcs = sorted(f for f in os.listdir(fdir) if f.endswith(".f90"))
+    if not srcs or not os.path.exists(fc):
+        return None
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    for s in srcs:  # module first (sorted: tara_hip_mod.f90 is compiled before users via order below)
+        pass
+    order = sorted(srcs, key=lambda s: (0 if s.endswith("_mod.f90") else 1, s))
+    for s in order:
+        o = os.path.join(OBJ, s[:-4] + ".fo")
+        _run([fc, "-O2", "-fPIC", "-J" + OBJ, "-c", os.path.join(fdir, s), "-o", o])
+        objs.append(o)
+    tmp = FLIB + ".tmp"
+    _run([fc, "-shared", "-o", tmp] + objs + ["-L" + HERE, "-ltara_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FLIB)
+    return FLIB
+
+
+def main():
+    build_hip()
+    build_fortran()
+    print(LIB)
+
+
+if __name__ == "__main__":
+    main()

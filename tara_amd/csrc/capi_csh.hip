@@ -1,0 +1,257 @@
+// C ABI of the 2D incompressible (csh) path: drop-in derive2csh / ab2csh on
+// host arrays, and the device-resident stepping context.
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+
+#include "common.h"
+#include "csh2d.h"
+
+namespace tara {
+
+void EventTimer::begin(int cls, hipStream_t st)
+{
+    Pair p;
+    if (!pool.empty()) {
+        p = pool.back();
+        pool.pop_back();
+    } else {
+        (void)hipEventCreate(&p.a);
+        (void)hipEventCreate(&p.b);
+    }
+    p.cls = cls;
+    (void)hipEventRecord(p.a, st);
+    cur[cls] = p;
+}
+
+void EventTimer::end(int cls, hipStream_t st)
+{
+    Pair p = cur[cls];
+    (void)hipEventRecord(p.b, st);
+    live.push_back(p);
+}
+
+int EventTimer::collect()
+{
+    for (auto& p : live) {
+        TARA_HIP(hipEventSynchronize(p.b));
+        float t = 0.f;
+        TARA_HIP(hipEventElapsedTime(&t, p.a, p.b));
+        counts[p.cls] += 1;
+        ms[p.cls] += t;
+        pool.push_back(p);
+    }
+    live.clear();
+    return TARA_OK;
+}
+
+EventTimer::~EventTimer()
+{
+    for (auto& p : live) pool.push_back(p);
+    for (auto& p : pool) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+}
+
+namespace {
+
+// Drop-in calls keep one plan and staging buffers per (device, nx, ny), as the
+// reference re-plans per call (src/derive2csh.f95:73-106) but need not.
+struct DropIn {
+    CshPlan plan;
+    DevBuf w, f;
+};
+std::mutex g_mu;
+std::map<std::tuple<int, int, int>, std::unique_ptr<DropIn>> g_dropin;
+
+int get_dropin(int nx, int ny, DropIn** out)
+{
+    auto key = std::make_tuple(current_device(), nx, ny);
+    auto it = g_dropin.find(key);
+    if (it != g_dropin.end()) {
+        *out = it->second.get();
+        return TARA_OK;
+    }
+    auto d = std::make_unique<DropIn>();
+    int rc = csh_plan_init(d->plan, nx, ny);
+    if (rc) return rc;
+    const size_t bytes = (size_t)d->plan.ld * ny * sizeof(double2);
+    if ((rc = d->w.alloc(bytes)) || (rc = d->f.alloc(bytes))) return rc;
+    TARA_HIP(hipMemset(d->w.p, 0, bytes));
+    *out = d.get();
+    g_dropin[key] = std::move(d);
+    return TARA_OK;
+}
+
+struct Ab2Bufs {
+    DevBuf b[4];
+    size_t bytes = 0;
+};
+Ab2Bufs g_ab2;
+
+}  // namespace
+}  // namespace tara
+
+using namespace tara;
+
+extern "C" int tara_derive2csh(int nx, int ny, int nh, double nu, const double* omegak, double* dt_new)
+{
+    TARA_CHECK(omegak && dt_new, TARA_ERR_ARG, "derive2csh: null array");
+    TARA_CHECK(nh == nx / 2 + 1, TARA_ERR_ARG, "derive2csh: Nh=%d but Nx/2+1=%d", nh, nx / 2 + 1);
+    std::lock_guard<std::mutex> lk(g_mu);
+    DropIn* d = nullptr;
+    int rc = get_dropin(nx, ny, &d);
+    if (rc) return rc;
+    const CshPlan& p = d->plan;
+    TARA_HIP(hipMemcpy2D(d->w.p, p.ld * sizeof(double2), omegak, nh * sizeof(double2), nh * sizeof(double2), ny,
+                         hipMemcpyHostToDevice));
+    rc = csh_rhs_stage(p, d->w.as<double2>(), CSH_RHS, nu, 0.0, nullptr, nullptr, d->f.as<double2>(), nullptr,
+                       nullptr);
+    if (rc) return rc;
+    TARA_HIP(hipMemcpy2D(dt_new, nh * sizeof(double2), d->f.p, p.ld * sizeof(double2), nh * sizeof(double2), ny,
+                         hipMemcpyDeviceToHost));
+    return TARA_OK;
+}
+
+extern "C" int tara_ab2csh(int nx, int ny, int nh, double dt, const double* omegak, double* omegak_new,
+                           const double* f_old, const double* f_new)
+{
+    TARA_CHECK(omegak && omegak_new && f_old && f_new, TARA_ERR_ARG, "ab2csh: null array");
+    TARA_CHECK(nh == nx / 2 + 1 && nx > 0 && ny > 0, TARA_ERR_ARG, "ab2csh: bad dims %d %d %d", nx, ny, nh);
+    std::lock_guard<std::mutex> lk(g_mu);
+    const size_t n = (size_t)nh * ny, bytes = n * sizeof(double2);
+    if (g_ab2.bytes < bytes) {
+        for (auto& b : g_ab2.b) {
+            int rc = b.alloc(bytes);
+            if (rc) return rc;
+        }
+        g_ab2.bytes = bytes;
+    }
+    TARA_HIP(hipMemcpy(g_ab2.b[0].p, omegak, bytes, hipMemcpyHostToDevice));
+    TARA_HIP(hipMemcpy(g_ab2.b[2].p, f_old, bytes, hipMemcpyHostToDevice));
+    TARA_HIP(hipMemcpy(g_ab2.b[3].p, f_new, bytes, hipMemcpyHostToDevice));
+    int rc = launch_ab2(g_ab2.b[0].as<double2>(), g_ab2.b[1].as<double2>(), g_ab2.b[2].as<double2>(),
+                        g_ab2.b[3].as<double2>(), dt, n, nullptr);
+    if (rc) return rc;
+    TARA_HIP(hipMemcpy(omegak_new, g_ab2.b[1].p, bytes, hipMemcpyDeviceToHost));
+    return TARA_OK;
+}
+
+// ------------------------------------------------------- stepping context
+struct tara_csh {
+    CshPlan plan;
+    DevBuf w, aux, next;
+    hipStream_t st = nullptr;
+    double nu = 0, dt = 0;
+    int scheme = 0;
+    std::unique_ptr<EventTimer> tm;
+};
+
+extern "C" int tara_csh_create(int nx, int ny, double nu, double dt, int scheme, tara_csh** out)
+{
+    TARA_CHECK(out, TARA_ERR_ARG, "null out");
+    TARA_CHECK(scheme == TARA_SCHEME_AB2 || scheme == TARA_SCHEME_RK4, TARA_ERR_ARG, "unknown scheme %d", scheme);
+    *out = nullptr;
+    auto c = std::make_unique<tara_csh>();
+    int rc = csh_plan_init(c->plan, nx, ny);
+    if (rc) return rc;
+    const size_t bytes = (size_t)c->plan.ld * ny * sizeof(double2);
+    if ((rc = c->w.alloc(bytes)) || (rc = c->aux.alloc(bytes))) return rc;
+    if (scheme == TARA_SCHEME_RK4 && (rc = c->next.alloc(bytes))) return rc;
+    TARA_HIP(hipMemset(c->w.p, 0, bytes));
+    TARA_HIP(hipMemset(c->aux.p, 0, bytes));
+    if (c->next.p) TARA_HIP(hipMemset(c->next.p, 0, bytes));
+    TARA_HIP(hipStreamCreate(&c->st));
+    c->nu = nu;
+    c->dt = dt;
+    c->scheme = scheme;
+    *out = c.release();
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_upload(tara_csh* c, const double* omegak)
+{
+    TARA_CHECK(c && omegak, TARA_ERR_ARG, "null argument");
+    const CshPlan& p = c->plan;
+    TARA_HIP(hipMemcpy2DAsync(c->w.p, p.ld * sizeof(double2), omegak, p.nh * sizeof(double2), p.nh * sizeof(double2),
+                              p.ny, hipMemcpyHostToDevice, c->st));
+    TARA_HIP(hipMemsetAsync(c->aux.p, 0, c->aux.bytes, c->st));
+    TARA_HIP(hipStreamSynchronize(c->st));
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_download(tara_csh* c, double* omegak)
+{
+    TARA_CHECK(c && omegak, TARA_ERR_ARG, "null argument");
+    const CshPlan& p = c->plan;
+    TARA_HIP(hipMemcpy2DAsync(omegak, p.nh * sizeof(double2), c->w.p, p.ld * sizeof(double2), p.nh * sizeof(double2),
+                              p.ny, hipMemcpyDeviceToHost, c->st));
+    TARA_HIP(hipStreamSynchronize(c->st));
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_advance(tara_csh* c, int nsteps)
+{
+    TARA_CHECK(c && nsteps >= 0, TARA_ERR_ARG, "bad argument");
+    double2* w = c->w.as<double2>();
+    double2* aux = c->aux.as<double2>();
+    double2* nx = c->next.as<double2>();
+    EventTimer* tm = c->tm.get();
+    for (int s = 0; s < nsteps; ++s) {
+        int rc;
+        if (c->scheme == TARA_SCHEME_AB2) {
+            rc = csh_rhs_stage(c->plan, w, CSH_AB2, c->nu, c->dt, w, aux, nullptr, c->st, tm);
+            if (rc) return rc;
+        } else {
+            if ((rc = csh_rhs_stage(c->plan, w, CSH_RK4_1, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
+            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_2, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
+            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_3, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
+            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_4, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
+        }
+    }
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_sync(tara_csh* c)
+{
+    TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    TARA_HIP(hipStreamSynchronize(c->st));
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_destroy(tara_csh* c)
+{
+    if (!c) return TARA_OK;
+    if (c->st) {
+        (void)hipStreamSynchronize(c->st);
+        c->tm.reset();
+        (void)hipStreamDestroy(c->st);
+    }
+    delete c;
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_set_timing(tara_csh* c, int on)
+{
+    TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    if (on && !c->tm) c->tm = std::make_unique<EventTimer>();
+    if (!on) c->tm.reset();
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_kernel_stats(tara_csh* c, int nclass, long* counts, double* ms)
+{
+    TARA_CHECK(c && c->tm, TARA_ERR_STATE, "timing not enabled");
+    int rc = c->tm->collect();
+    if (rc) return rc;
+    for (int k = 0; k < nclass && k < EventTimer::kClasses; ++k) {
+        if (counts) counts[k] = c->tm->counts[k];
+        if (ms) ms[k] = c->tm->ms[k];
+        c->tm->counts[k] = 0;
+        c->tm->ms[k] = 0;
+    }
+    return TARA_OK;
+}

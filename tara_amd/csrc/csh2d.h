@@ -1,0 +1,49 @@
+// Host interface of the 2D incompressible (csh) pipeline.
+#pragma once
+#include <vector>
+
+#include "common.h"
+
+namespace tara {
+
+enum CshMode { CSH_RHS = 0, CSH_AB2 = 1, CSH_RK4_1 = 2, CSH_RK4_2 = 3, CSH_RK4_3 = 4, CSH_RK4_4 = 5 };
+
+// Brackets launches with HIP events on the launch stream, per kernel class.
+// Events are recycled from a pool; durations are summed when collected.
+struct EventTimer {
+    static constexpr int kClasses = 8;
+    struct Pair {
+        hipEvent_t a, b;
+        int cls;
+    };
+    std::vector<Pair> live;
+    std::vector<Pair> pool;
+    Pair cur[kClasses];
+    long counts[kClasses] = {0};
+    double ms[kClasses] = {0};
+    void begin(int cls, hipStream_t st);
+    void end(int cls, hipStream_t st);
+    int collect();  // synchronises the recorded events, accumulates counts/ms
+    ~EventTimer();
+};
+
+// Plan + scratch of one (nx, ny) grid: padded leading dimension, tables,
+// intermediate fields G0..G3 (after the inverse y pass) and H (after x).
+struct CshPlan {
+    int nx = 0, ny = 0, nh = 0, ld = 0;
+    DevBuf g[4], h;
+    DevBuf kx, ky, imin;
+    const double2* tw_y = nullptr;
+    const double2* tw_x = nullptr;
+    const double2* rctw = nullptr;
+};
+
+int csh_plan_init(CshPlan& p, int nx, int ny);
+// One right-hand-side evaluation on stage input ws followed by the fused
+// update selected by mode (see k_csh_yfwd).
+int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, double nu, double dt, double2* base, double2* aux,
+                  double2* next, hipStream_t st, EventTimer* tm);
+int launch_ab2(const double2* w, double2* wn, const double2* fo, const double2* fn, double dt, size_t n,
+               hipStream_t st);
+
+}  // namespace tara

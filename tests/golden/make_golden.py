@@ -1,0 +1,106 @@
+"""Writes the golden fixtures under tests/golden/ from the REFERENCE build
+(oracle/_ref/libtara_ref.so: TARA's own Fortran routines + the vendored FFTW
+3.3.9, compiled from /root/reference by `make -C oracle ref`).
+
+The reference has no tests, fixtures or golden files of its own; these vectors
+are produced by running its routines on the inputs below.  Run here (where
+/root/reference exists); the .npz files travel with the repository, the
+reference does not.
+
+    ulimit -s unlimited && python tests/golden/make_golden.py
+
+(derive3com keeps ~130 automatic arrays on the stack, hence the stack limit.)
+"""
+import os
+import platform
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+import oracle as O  # noqa: E402
+
+PI = 3.14159265358979323846
+
+
+def smooth_field(rng, n, nmodes=6, kmax=3):
+    x = np.arange(n) * 2 * PI / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.zeros((n, n, n))
+    for _ in range(nmodes):
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        f += rng.standard_normal() * np.cos(k[0] * X + k[1] * Y + k[2] * Z + rng.uniform(0, 2 * PI))
+    return f
+
+
+def mhd_state(seed, n):
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.1 * smooth_field(rng, n) / 6
+    u = [0.1 * smooth_field(rng, n) for _ in range(3)]
+    B = [0.1 * smooth_field(rng, n) for _ in range(3)]
+    E = np.ones((n, n, n))
+    reals = [rho, rho * u[0], rho * u[1], rho * u[2], E, B[0], B[1], B[2]]
+    return [O.r2c(np.asfortranarray(r)) for r in reals]
+
+
+def random_spec(seed, n):
+    rng = np.random.default_rng(seed)
+    return O.r2c(np.asfortranarray(rng.standard_normal((n, n))))
+
+
+def main():
+    meta = dict(
+        generator="tests/golden/make_golden.py",
+        reference="RupakMukherjee/TARA src/*.f95 + lib/fftw 3.3.9 (oracle/Makefile)",
+        fc=subprocess.run(["amdflang", "--version"], capture_output=True, text=True).stdout.splitlines()[0],
+        cc=subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0],
+        host=platform.machine(),
+    )
+    out = {}
+    # Config-1 trajectory (src/main2csh.f95 loop), 64^2, nu = 0.01, dt = 1e-4,
+    # shear layer W0 = 25, m0 = 2; d as-run (3.0 after list-directed parsing of
+    # "3.0d0/128.0d0") and the intended 3/128.  History starts at zero.
+    for tag, d in (("asrun", 3.0), ("intent", 3.0 / 128.0)):
+        omega, wk = O.ref_ic2csh(64, 64, 2.0, 2.0, 25.0, 2.0, d)
+        out[f"c1_{tag}_omega0"] = omega
+        out[f"c1_{tag}_w0"] = wk
+        w, h = wk, O.zeros_spec(64, 64)
+        done = 0
+        for target in (1, 1000, 2000):
+            w, h = O.ref_ab2csh_steps(w, h, 64, 0.01, 1e-4, target - done)
+            done = target
+            out[f"c1_{tag}_w{target}"] = w
+        out[f"c1_{tag}_hist2000"] = h
+    # RK4 over derive2csh (harness), 64^2 shear layer, 200 steps.
+    out["rk4_w0"] = out["c1_asrun_w0"]
+    out["rk4_w200"] = O.ref_rk4csh_steps(out["rk4_w0"], 64, 0.01, 1e-4, 200)
+    # Single derive2csh calls.
+    for n in (64, 128):
+        _, ic = O.ref_ic2csh(n, n, 2.0, 2.0, 25.0, 2.0, 3.0 / 128.0)
+        rnd = random_spec(100 + n, n)
+        out[f"d2csh_{n}_ic_in"] = ic
+        out[f"d2csh_{n}_ic_out"] = O.ref_derive2csh(ic, n, 0.01)
+        out[f"d2csh_{n}_rnd_in"] = rnd
+        out[f"d2csh_{n}_rnd_out"] = O.ref_derive2csh(rnd, n, 1e-3)
+    # Single ab2csh call.
+    a, b, c = random_spec(1, 64), random_spec(2, 64), random_spec(3, 64)
+    out["ab2_in_w"], out["ab2_in_fo"], out["ab2_in_fn"] = a, b, c
+    out["ab2_out"] = O.ref_ab2csh(a, b, c, 64, 1e-3)
+    # Single derive3com call at 16^3 (diffusion 1/450 as hard-wired, CS = 10).
+    st = mhd_state(7, 16)
+    f = O.ref_derive3com(st, 16, 2 * PI, 2 * PI, 2 * PI, cs=10.0)
+    for k in range(8):
+        out[f"d3com_in{k}"] = st[k]
+        if k != 4:
+            out[f"d3com_out{k}"] = f[k]
+    np.savez_compressed(os.path.join(HERE, "golden.npz"), **out)
+    with open(os.path.join(HERE, "golden_meta.txt"), "w") as fh:
+        for k, v in meta.items():
+            fh.write(f"{k}: {v}\n")
+    print("wrote", len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+"""GPU parity of the 2D path, through the C ABI (tara_amd -> libtara_hip.so).
+
+Bar: relative L2 <= 1e-10 against the reference trajectory (north-star
+tolerance, FP64); single calls and transforms are held to 1e-13 / 1e-14, which
+is the rounding level of two different FFT factorisations.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import tara_amd as T
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+PI = np.pi
+
+
+def _rand_real(shape, seed):
+    return np.asfortranarray(np.random.default_rng(seed).standard_normal(shape))
+
+
+# ------------------------------------------------------------------ transforms
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (128, 256), (1024, 512), (4096, 64), (64, 4096),
+                                   (2048, 128), (8192, 32)])
+def test_fft_2d_vs_oracle(nx, ny):
+    print(f"fft2d {nx}x{ny}", flush=True)
+    x = _rand_real((nx, ny), nx + 7 * ny)
+    spec = T.fft_r2c(x)
+    assert rel_l2(spec, O.r2c(x)) < 1e-14
+    rng = np.random.default_rng(1)
+    s = np.asfortranarray(spec + 0.1 * np.linalg.norm(spec) / np.sqrt(spec.size) *
+                          (rng.standard_normal(spec.shape) + 1j * rng.standard_normal(spec.shape)))
+    before = s.copy()
+    got = T.fft_c2r(s, nx)
+    assert np.array_equal(s, before), "c2r must not modify its input"
+    assert rel_l2(got, O.c2r(s, nx)) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 64, 16), (64, 64, 64), (256, 16, 32)])
+def test_fft_3d_vs_oracle(shape):
+    print(f"fft3d {shape}", flush=True)
+    x = _rand_real(shape, sum(shape))
+    spec = T.fft_r2c(x)
+    assert rel_l2(spec, O.r2c(x)) < 1e-14
+    assert rel_l2(T.fft_c2r(spec, shape[0]), O.c2r(spec, shape[0])) < 1e-14
+
+
+# ------------------------------------------------------------------ derive2csh
+@pytest.mark.parametrize("n", [64, 128])
+@pytest.mark.parametrize("case,nu", [("ic", 0.01), ("rnd", 1e-3)])
+def test_derive2csh_vs_reference(golden, n, case, nu):
+    w = golden[f"d2csh_{n}_{case}_in"]
+    before = w.copy()
+    got = T.derive2csh(n, n, n // 2 + 1, nu, w)
+    assert np.array_equal(w, before)
+    assert rel_l2(got, golden[f"d2csh_{n}_{case}_out"]) < 1e-13
+
+
+@pytest.mark.parametrize("nx,ny", [(256, 256), (512, 128), (128, 512), (1024, 1024), (2048, 256)])
+def test_derive2csh_vs_oracle(nx, ny):
+    print(f"derive2csh {nx}x{ny}", flush=True)
+    w = O.r2c(_rand_real((nx, ny), nx * 3 + ny))
+    got = T.derive2csh(nx, ny, nx // 2 + 1, 1e-3, w)
+    assert rel_l2(got, O.derive2csh(w, nx, 1e-3)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [64, 4096])
+def test_derive2csh_closed_form(n):
+    """omega = cos x + cos 2y -> f = 1.5 sin x sin 2y (nu = 0), at every size."""
+    print(f"closed form {n}", flush=True)
+    x = np.arange(n) * 2 * PI / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    w = T.fft_r2c(np.cos(X) + np.cos(2 * Y))
+    f = T.derive2csh(n, n, n // 2 + 1, 0.0, w)
+    fr = T.fft_c2r(f, n) / float(n * n)
+    assert np.max(np.abs(fr - 1.5 * np.sin(X) * np.sin(2 * Y))) < 1e-11
+
+
+def test_derive2csh_mask_and_enstrophy_4096():
+    """Full-size invariants: every mode beyond the dealias circle is exactly zero
+    (nu = 0), and for a field band-limited inside it the nonlinear term conserves
+    enstrophy (sum Re(conj(w) NL) = 0)."""
+    n = 4096
+    rng = np.random.default_rng(11)
+    kx = np.arange(n // 2 + 1)[:, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    w = np.zeros((n // 2 + 1, n), np.complex128, order="F")
+    sel = (k > 0) & (k < 60)
+    w[sel] = (rng.standard_normal(sel.sum()) + 1j * rng.standard_normal(sel.sum())) / (1 + k[sel]) ** 2
+    w[0, :] = 0  # keep the kx = 0 column trivially Hermitian
+    f = T.derive2csh(n, n, n // 2 + 1, 0.0, w)
+    assert np.all(f[k >= (2 * n / 2.0) / 3.0] == 0)
+    wt = np.where(kx == 0, 1.0, 2.0)
+    ens = np.sum(wt * np.real(np.conj(w) * f))
+    assert abs(ens) / np.sum(wt * np.abs(w) * np.abs(f)) < 1e-12
+
+
+# ------------------------------------------------------------------ updates
+def test_ab2csh_vs_reference(golden):
+    got = T.ab2csh(64, 64, 33, 1e-3, golden["ab2_in_w"], golden["ab2_in_fo"], golden["ab2_in_fn"])
+    assert rel_l2(got, golden["ab2_out"]) < 1e-15
+
+
+@pytest.mark.parametrize("tag", ["asrun", "intent"])
+def test_config1_dropin_trajectory(golden, tag):
+    """The main2csh.f95:114-126 loop driven through the drop-in calls:
+    derive2csh + ab2csh + host-side rotation, 2000 steps, vs the reference."""
+    w = golden[f"c1_{tag}_w0"].copy(order="F")
+    hist = np.zeros_like(w, order="F")
+    for t in range(2000):
+        f = T.derive2csh(64, 64, 33, 0.01, w)
+        w = T.ab2csh(64, 64, 33, 1e-4, w, hist, f)
+        hist = f
+        if t + 1 == 1000:
+            assert rel_l2(w, golden[f"c1_{tag}_w1000"]) < 1e-12
+    assert rel_l2(w, golden[f"c1_{tag}_w2000"]) < 1e-10
+    assert rel_l2(hist, golden[f"c1_{tag}_hist2000"]) < 1e-10
+
+
+@pytest.mark.parametrize("tag", ["asrun", "intent"])
+def test_config1_resident_ab2(golden, tag):
+    s = T.CshSolver(64, 64, 0.01, 1e-4, T.SCHEME_AB2)
+    s.upload(golden[f"c1_{tag}_w0"])
+    s.advance(1000)
+    assert rel_l2(s.download(), golden[f"c1_{tag}_w1000"]) < 1e-12
+    s.advance(1000)
+    assert rel_l2(s.download(), golden[f"c1_{tag}_w2000"]) < 1e-10
+    s.close()
+
+
+def test_rk4_resident_vs_reference(golden):
+    s = T.CshSolver(64, 64, 0.01, 1e-4, T.SCHEME_RK4)
+    s.upload(golden["rk4_w0"])
+    s.advance(200)
+    assert rel_l2(s.download(), golden["rk4_w200"]) < 1e-12
+    s.close()
+
+
+@pytest.mark.parametrize("nx,ny,steps", [(256, 256, 20), (1024, 1024, 3), (512, 2048, 2)])
+def test_rk4_resident_vs_oracle(nx, ny, steps):
+    print(f"rk4 {nx}x{ny}", flush=True)
+    x = np.arange(nx) * 2 * PI / nx
+    y = np.arange(ny) * 2 * PI / ny
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    w0 = T.fft_r2c(np.sin(3 * X) * np.cos(2 * Y) + 0.5 * np.cos(5 * X + Y) + 0.2 * np.sin(X - 4 * Y))
+    s = T.CshSolver(nx, ny, 1e-3, 2e-3, T.SCHEME_RK4)
+    s.upload(w0)
+    s.advance(steps)
+    got = s.download()
+    s.close()
+    assert rel_l2(got, O.rk4csh_steps(w0, nx, 1e-3, 2e-3, steps)) < 1e-12
+
+
+def test_rk4_4096_one_step_consistency():
+    """At 4096^2 (the benchmark size): one device-resident RK4 step equals the
+    same step composed from four drop-in derive2csh calls and host stage updates."""
+    n, nu, dt = 4096, 1e-4, 2e-4
+    rng = np.random.default_rng(4)
+    kx = np.arange(n // 2 + 1)[:, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    w = np.zeros((n // 2 + 1, n), np.complex128, order="F")
+    sel = (k > 0) & (k < 40)
+    w[sel] = (rng.standard_normal(sel.sum()) + 1j * rng.standard_normal(sel.sum())) * k[sel] ** 2 * np.exp(-(k[sel] / 4) ** 2)
+    w[0, :] = 0
+    s = T.CshSolver(n, n, nu, dt, T.SCHEME_RK4)
+    s.upload(w)
+    s.advance(1)
+    got = s.download()
+    s.close()
+    d = lambda a: T.derive2csh(n, n, n // 2 + 1, nu, a)
+    k1 = d(w)
+    k2 = d(w + 0.5 * dt * k1)
+    k3 = d(w + 0.5 * dt * k2)
+    k4 = d(w + dt * k3)
+    ref = w + (dt / 6.0) * (k1 + 2 * k2 + 2 * k3 + k4)
+    assert rel_l2(got, ref) < 1e-14
