@@ -257,6 +257,7 @@ static int launch_yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_
 
 #define TARA_SIZE_SWITCH(n, FN, ...)                \
     switch (n) {                                    \
+        case 8: return FN<8>(__VA_ARGS__);          \
         case 16: return FN<16>(__VA_ARGS__);        \
         case 32: return FN<32>(__VA_ARGS__);        \
         case 64: return FN<64>(__VA_ARGS__);        \

@@ -97,6 +97,7 @@ __global__ __launch_bounds__(RowShape<M>::NT) void k_row_c2r(const double2* __re
 
 #define TARA_LEN_SWITCH(n, FN, ...)                 \
     switch (n) {                                    \
+        case 8: return FN<8>(__VA_ARGS__);          \
         case 16: return FN<16>(__VA_ARGS__);        \
         case 32: return FN<32>(__VA_ARGS__);        \
         case 64: return FN<64>(__VA_ARGS__);        \
@@ -172,9 +173,9 @@ using namespace tara;
 namespace {
 int check_dims(int nx, int ny, int nz)
 {
-    TARA_CHECK(is_pow2(nx) && is_pow2(ny) && (nz == 1 || is_pow2(nz)) && nx >= 32 && nx <= 8192 && ny >= 16 &&
+    TARA_CHECK(is_pow2(nx) && is_pow2(ny) && (nz == 1 || is_pow2(nz)) && nx >= 16 && nx <= 8192 && ny >= 8 &&
                    ny <= 4096 && nz <= 4096,
-               TARA_ERR_UNSUPPORTED, "transform size %dx%dx%d unsupported (powers of two; nx 32..8192, ny, nz <= 4096)",
+               TARA_ERR_UNSUPPORTED, "transform size %dx%dx%d unsupported (powers of two; nx 16..8192, ny, nz 8..4096)",
                nx, ny, nz);
     return TARA_OK;
 }
