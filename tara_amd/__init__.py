@@ -18,7 +18,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtara_hip.so")
+LIB_PATH = os.environ.get("TARA_HIP_LIB") or os.path.join(_HERE, "libtara_hip.so")
 _dp = C.POINTER(C.c_double)
 _lib = None
 

@@ -8,6 +8,7 @@
 
 #include "common.h"
 #include "csh2d.h"
+#include "layout.h"
 
 namespace tara {
 
@@ -62,7 +63,7 @@ namespace {
 // reference re-plans per call (src/derive2csh.f95:73-106) but need not.
 struct DropIn {
     CshPlan plan;
-    DevBuf w, f;
+    DevBuf w, f, stage;  // w, f ky-major; stage in the caller's kx-fastest layout
 };
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, std::unique_ptr<DropIn>> g_dropin;
@@ -78,9 +79,8 @@ int get_dropin(int nx, int ny, DropIn** out)
     auto d = std::make_unique<DropIn>();
     int rc = csh_plan_init(d->plan, nx, ny);
     if (rc) return rc;
-    const size_t bytes = (size_t)d->plan.ld * ny * sizeof(double2);
-    if ((rc = d->w.alloc(bytes)) || (rc = d->f.alloc(bytes))) return rc;
-    TARA_HIP(hipMemset(d->w.p, 0, bytes));
+    const size_t bytes = (size_t)d->plan.nh * ny * sizeof(double2);
+    if ((rc = d->w.alloc(bytes)) || (rc = d->f.alloc(bytes)) || (rc = d->stage.alloc(bytes))) return rc;
     *out = d.get();
     g_dropin[key] = std::move(d);
     return TARA_OK;
@@ -106,13 +106,14 @@ extern "C" int tara_derive2csh(int nx, int ny, int nh, double nu, const double* 
     int rc = get_dropin(nx, ny, &d);
     if (rc) return rc;
     const CshPlan& p = d->plan;
-    TARA_HIP(hipMemcpy2D(d->w.p, p.ld * sizeof(double2), omegak, nh * sizeof(double2), nh * sizeof(double2), ny,
-                         hipMemcpyHostToDevice));
+    const size_t bytes = (size_t)nh * ny * sizeof(double2);
+    TARA_HIP(hipMemcpy(d->stage.p, omegak, bytes, hipMemcpyHostToDevice));
+    if ((rc = to_kymajor(d->stage.as<double2>(), d->w.as<double2>(), nh, ny, 1, nullptr))) return rc;
     rc = csh_rhs_stage(p, d->w.as<double2>(), CSH_RHS, nu, 0.0, nullptr, nullptr, d->f.as<double2>(), nullptr,
                        nullptr);
     if (rc) return rc;
-    TARA_HIP(hipMemcpy2D(dt_new, nh * sizeof(double2), d->f.p, p.ld * sizeof(double2), nh * sizeof(double2), ny,
-                         hipMemcpyDeviceToHost));
+    if ((rc = from_kymajor(d->f.as<double2>(), d->stage.as<double2>(), nh, ny, 1, nullptr))) return rc;
+    TARA_HIP(hipMemcpy(dt_new, d->stage.p, bytes, hipMemcpyDeviceToHost));
     return TARA_OK;
 }
 
@@ -143,7 +144,7 @@ extern "C" int tara_ab2csh(int nx, int ny, int nh, double dt, const double* omeg
 // ------------------------------------------------------- stepping context
 struct tara_csh {
     CshPlan plan;
-    DevBuf w, aux, next;
+    DevBuf w, aux, next, stage;  // ky-major state; stage in the caller's layout
     hipStream_t st = nullptr;
     double nu = 0, dt = 0;
     int scheme = 0;
@@ -158,8 +159,8 @@ extern "C" int tara_csh_create(int nx, int ny, double nu, double dt, int scheme,
     auto c = std::make_unique<tara_csh>();
     int rc = csh_plan_init(c->plan, nx, ny);
     if (rc) return rc;
-    const size_t bytes = (size_t)c->plan.ld * ny * sizeof(double2);
-    if ((rc = c->w.alloc(bytes)) || (rc = c->aux.alloc(bytes))) return rc;
+    const size_t bytes = (size_t)c->plan.nh * ny * sizeof(double2);
+    if ((rc = c->w.alloc(bytes)) || (rc = c->aux.alloc(bytes)) || (rc = c->stage.alloc(bytes))) return rc;
     if (scheme == TARA_SCHEME_RK4 && (rc = c->next.alloc(bytes))) return rc;
     TARA_HIP(hipMemset(c->w.p, 0, bytes));
     TARA_HIP(hipMemset(c->aux.p, 0, bytes));
@@ -176,8 +177,9 @@ extern "C" int tara_csh_upload(tara_csh* c, const double* omegak)
 {
     TARA_CHECK(c && omegak, TARA_ERR_ARG, "null argument");
     const CshPlan& p = c->plan;
-    TARA_HIP(hipMemcpy2DAsync(c->w.p, p.ld * sizeof(double2), omegak, p.nh * sizeof(double2), p.nh * sizeof(double2),
-                              p.ny, hipMemcpyHostToDevice, c->st));
+    TARA_HIP(hipMemcpyAsync(c->stage.p, omegak, c->stage.bytes, hipMemcpyHostToDevice, c->st));
+    int rc = to_kymajor(c->stage.as<double2>(), c->w.as<double2>(), p.nh, p.ny, 1, c->st);
+    if (rc) return rc;
     TARA_HIP(hipMemsetAsync(c->aux.p, 0, c->aux.bytes, c->st));
     TARA_HIP(hipStreamSynchronize(c->st));
     return TARA_OK;
@@ -187,8 +189,9 @@ extern "C" int tara_csh_download(tara_csh* c, double* omegak)
 {
     TARA_CHECK(c && omegak, TARA_ERR_ARG, "null argument");
     const CshPlan& p = c->plan;
-    TARA_HIP(hipMemcpy2DAsync(omegak, p.nh * sizeof(double2), c->w.p, p.ld * sizeof(double2), p.nh * sizeof(double2),
-                              p.ny, hipMemcpyDeviceToHost, c->st));
+    int rc = from_kymajor(c->w.as<double2>(), c->stage.as<double2>(), p.nh, p.ny, 1, c->st);
+    if (rc) return rc;
+    TARA_HIP(hipMemcpyAsync(omegak, c->stage.p, c->stage.bytes, hipMemcpyDeviceToHost, c->st));
     TARA_HIP(hipStreamSynchronize(c->st));
     return TARA_OK;
 }

@@ -27,10 +27,10 @@ struct EventTimer {
     ~EventTimer();
 };
 
-// Plan + scratch of one (nx, ny) grid: padded leading dimension, tables,
-// intermediate fields G0..G3 (after the inverse y pass) and H (after x).
+// Plan + scratch of one (nx, ny) grid: tables and the intermediate fields
+// G0..G3 (after the inverse y pass) and H (after x), all ky-major (nh x ny).
 struct CshPlan {
-    int nx = 0, ny = 0, nh = 0, ld = 0;
+    int nx = 0, ny = 0, nh = 0;
     DevBuf g[4], h;
     DevBuf kx, ky, imin;
     const double2* tw_y = nullptr;

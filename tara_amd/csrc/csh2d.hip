@@ -2,145 +2,173 @@
 // the right-hand side of src/derive2csh.f95 and the AB2 / RK4 updates
 // (src/ab2csh.f95; classic RK4), as three fused HBM sweeps per RHS.
 //
-//   pass 1  k_csh_yinv : per column tile (TC kx columns x Ny) read omega_hat,
-//                        form the four spectra of derive2csh.f95:29-71
-//                        (i ky psi, -i kx psi, i kx omega, i ky omega; psi = omega/k^2,
-//                        psi(0,0) = 0) and run the backward y FFTs -> G0..G3.
-//   pass 2  k_csh_x    : per row, four x c2r (half-length packing), normalise by
-//                        Nx*Ny and form ux*d_x omega + uy*d_y omega
-//                        (derive2csh.f95:89-98), one x r2c -> H.
-//                        (The reference transforms the two products separately
-//                        (:100-106); the sum is transformed once, which is the
-//                        same linear map.)
-//   pass 3  k_csh_yfwd : per column tile, forward y FFT of H, circular dealias
+// Device layout: every half spectrum is stored ky-major ("transposed"),
+// element (kx = i, ky = j) at i*Ny + j, so a kx column (one y transform) is a
+// contiguous Ny-element run.  Both y passes then stream whole columns; the x
+// pass, which needs whole rows, reads and writes RB-row chunks (RB*16 bytes
+// contiguous per kx).  The reference's (Nh, Ny) kx-fastest arrays are converted
+// on upload / download (layout.hip).
+//
+//   pass 1  k_csh_yinv : per kx column read omega_hat, form the four spectra of
+//                        derive2csh.f95:29-71 (i kx omega, i ky omega, i ky psi,
+//                        -i kx psi; psi = omega/k^2, psi(0,0) = 0) and run the
+//                        backward y FFTs -> G0..G3.
+//   pass 2  k_csh_x    : per RB rows, four x c2r (half-length packing),
+//                        normalise by Nx*Ny and form ux*d_x omega + uy*d_y omega
+//                        (derive2csh.f95:89-98), one x r2c -> H.  (The reference
+//                        transforms the two products separately (:100-106); the
+//                        sum is transformed once, which is the same linear map.)
+//   pass 3  k_csh_yfwd : per kx column, forward y FFT of H, circular dealias
 //                        sqrt(kx^2+ky^2) >= (Nx+Ny)/6 on the nonlinear part only
-//                        (:108-125), add nu k^2 omega (:127-139), negate (:141-145),
-//                        then the fused update (RHS out / AB2 / RK4 stage).
+//                        (:108-125), add nu k^2 omega (:127-139), negate
+//                        (:141-145), then the fused update (RHS / AB2 / RK4 stage).
 //
 // Wavenumbers: derive2csh uses pi = 3.1428 (a default-real literal) and
 // Lx = Ly = 2 pi (:16-21), so kx = i-1 and ky = j-1 or j-1-Ny are exact integers;
-// the host builds the tables with that exact expression (csh_tables).
+// the host builds the tables with that exact expression (csh_plan_init).
 #include "common.h"
 #include "csh2d.h"
 #include "passes.h"
+#include "rfft.h"
 
 namespace tara {
 
+// Columns per workgroup of the y passes: at least 256 threads.
+template <int N>
+struct YShape {
+    static constexpr int T = RF<N>::T;
+    static constexpr int CB = T >= 256 ? 1 : 256 / T;
+    static constexpr int NT = CB * T;
+};
+// Rows per workgroup of the x pass: 512 threads where that fits (RB = 4 rows,
+// 64 contiguous bytes per kx, at Nx = 4096), at most 8 rows.
+#ifndef TARA_X_THREADS
+#define TARA_X_THREADS 512
+#endif
+#ifndef TARA_X_WAVES
+#define TARA_X_WAVES 2
+#endif
+template <int M>
+struct XShape {
+    static constexpr int T = RF<M>::T;
+    static constexpr int RB = (TARA_X_THREADS / T) < 2 ? 2 : (TARA_X_THREADS / T) > 8 ? 8 : (TARA_X_THREADS / T);
+    static constexpr int NT = RB * T;
+    static constexpr int MINB = TARA_X_WAVES;  // waves per SIMD the register budget is sized for
+    static_assert(RB * RF<M>::LDS * 16 <= 160 * 1024, "x pass exchange buffer exceeds LDS");
+};
+
+// derive2csh's ky: j-1 for j <= Ny/2, j-1-Ny above (exact integers, checked on the host).
+template <int NY>
+__device__ __forceinline__ double csh_ky(int j)
+{
+    return (double)(j < NY / 2 ? j : j - NY);
+}
+
 // --------------------------------------------------------------------- pass 1
 template <int NY>
-__global__ __launch_bounds__(ColShape<NY>::NT) void k_csh_yinv(const double2* __restrict__ w, int ld,
-                                                                const double* __restrict__ kx,
-                                                                const double* __restrict__ ky,
-                                                                double2* __restrict__ g0, double2* __restrict__ g1,
-                                                                double2* __restrict__ g2, double2* __restrict__ g3,
-                                                                const double2* __restrict__ tw)
+__global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, int nh,
+                                                              const double* kx,
+                                                              const double* ky,
+                                                              double2* __restrict__ g0, double2* __restrict__ g1,
+                                                              double2* __restrict__ g2, double2* __restrict__ g3,
+                                                              const double2* tw)
 {
-    using S = ColShape<NY>;
-    constexpr int TC = S::TC, NT = S::NT, PER = S::PER;
-    __shared__ double2 lds[TC * NY];
-    const int i0 = blockIdx.x * TC;
-    double2 wv[PER];
+    using S = YShape<NY>;
+    constexpr int E = RF<NY>::E, T = S::T;
+    __shared__ double2 lds[S::CB * RF<NY>::LDS];
+    const int c = threadIdx.x / T, t = threadIdx.x % T;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * S::CB + c;
+    const bool live = i < nh;
+    double2* x = lds + c * RF<NY>::LDS;
+    const size_t col = (size_t)(live ? i : 0) * NY;
+    double2 wv[E];
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = threadIdx.x + m * NT, j = e / TC, c = e % TC;
-        wv[m] = w[(size_t)j * ld + i0 + c];
-    }
-    // i kx omega and i ky omega first, then omega -> psi in place, then the
-    // velocity spectra +i ky psi and -i kx psi.
-    const double kxv = kx[i0 + threadIdx.x % TC];  // NT % TC == 0: one column per thread
+    for (int e = 0; e < E; ++e) wv[e] = live ? w[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
+    const double kxv = (double)i;  // derive2csh's kx = i-1 exactly (checked on the host)
+    // i kx omega, i ky omega first; then omega -> psi in place; then +i ky psi, -i kx psi.
 #pragma unroll 1
     for (int f = 0; f < 4; ++f) {
+        double2 v[E];
         if (f == 2) {
 #pragma unroll
-            for (int m = 0; m < PER; ++m) {
-                const int j = (threadIdx.x + m * NT) / TC;
-                const double kyv = ky[j];
+            for (int e = 0; e < E; ++e) {
+                const double kyv = csh_ky<NY>(rf_first_idx<NY>(t, e));
                 const double k2 = kxv * kxv + kyv * kyv;
-                wv[m] = (k2 != 0.0) ? make_double2(wv[m].x / k2, wv[m].y / k2) : make_double2(0.0, 0.0);
+                wv[e] = (k2 != 0.0) ? make_double2(wv[e].x / k2, wv[e].y / k2) : make_double2(0.0, 0.0);
             }
         }
 #pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = threadIdx.x + m * NT, j = e / TC;
-            const double kyv = ky[j];
-            const double2 z = wv[m];
-            double2 v;
-            if (f == 0)
-                v = make_double2(-(kxv * z.y), kxv * z.x);      // i kx omega
-            else if (f == 1)
-                v = make_double2(-(kyv * z.y), kyv * z.x);      // i ky omega
-            else if (f == 2)
-                v = make_double2(-(kyv * z.y), kyv * z.x);      // +i ky psi
-            else
-                v = make_double2(kxv * z.y, -(kxv * z.x));      // -i kx psi
-            lds[e] = v;
+        for (int e = 0; e < E; ++e) {
+            const double kyv = csh_ky<NY>(rf_first_idx<NY>(t, e));
+            const double kk = (f == 0 || f == 3) ? kxv : kyv;
+            const double2 z = wv[e];
+            v[e] = (f == 3) ? make_double2(kk * z.y, -(kk * z.x))     // -i kx psi
+                            : make_double2(-(kk * z.y), kk * z.x);    // i k (omega | psi)
         }
-        __syncthreads();
-        lds_fft<NY, +1, NT, TC, true>(lds, TC, 1, tw);
-        // G0 = ux, G1 = uy, G2 = d omega/dx, G3 = d omega/dy
+        rf_fft<NY, +1>(v, t, x, tw);
+        // G0 = ux (f 2), G1 = uy (f 3), G2 = d omega/dx (f 0), G3 = d omega/dy (f 1)
         double2* o = f == 0 ? g2 : f == 1 ? g3 : f == 2 ? g0 : g1;
+        if (live) {
 #pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = threadIdx.x + m * NT, j = e / TC, c = e % TC;
-            o[(size_t)j * ld + i0 + c] = lds[e];
+            for (int e = 0; e < E; ++e) o[col + rf_final_idx<NY>(t, e)] = v[e];
         }
-        __syncthreads();
     }
 }
 
 // --------------------------------------------------------------------- pass 2
+// x c2r of one field for row j: registers end up holding z at rf_final_idx
+// positions m, z[m] = x[2m] + i x[2m+1].
 template <int M>
-__global__ __launch_bounds__(RowShape<M>::NT) void k_csh_x(const double2* __restrict__ g0,
-                                                            const double2* __restrict__ g1,
-                                                            const double2* __restrict__ g2,
-                                                            const double2* __restrict__ g3, int ld, int nh,
-                                                            double scale, double2* __restrict__ h,
-                                                            const double2* __restrict__ tw,
-                                                            const double2* __restrict__ rctw)
+__device__ __forceinline__ void x_c2r(const double2* g, int ny, int j, int t, double2* xl,
+                                      const double2* tw, const double2* rctw,
+                                      double2 (&v)[RF<M>::E])
 {
-    using S = RowShape<M>;
-    constexpr int RB = S::RB, NT = S::NT, PER = S::PER, P = S::PITCH;
-    __shared__ double2 lds[RB * P];
-    const int row0 = blockIdx.x * RB;
-    const double2* gin[4] = {g0, g1, g2, g3};
-    double2 z[4][PER];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = threadIdx.x + m * NT, r = e / M, k = e % M;
-            const double2* src = gin[f] + (size_t)(row0 + r) * ld;
-            lds[r * P + k] = c2r_pack(src[k], src[M - k], k, rctw);
-        }
-        __syncthreads();
-        lds_fft<M, +1, NT, RB, false>(lds, 1, P, tw);
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = threadIdx.x + m * NT, r = e / M, k = e % M;
-            z[f][m] = lds[r * P + k];
-        }
-        __syncthreads();
+    for (int e = 0; e < RF<M>::E; ++e) {
+        const int k = rf_first_idx<M>(t, e);
+        v[e] = c2r_pack(g[(size_t)k * ny + j], g[(size_t)(M - k) * ny + j], k, rctw);
     }
-    // real-space products at x = 2k (real parts) and x = 2k+1 (imaginary parts)
+    rf_fft<M, +1>(v, t, xl, tw);
+}
+
+template <int M>
+__global__ __launch_bounds__(XShape<M>::NT, XShape<M>::MINB) void k_csh_x(const double2* g0,
+                                                          const double2* g1,
+                                                          const double2* g2,
+                                                          const double2* g3, int ny, double scale,
+                                                          double2* h, const double2* tw,
+                                                          const double2* rctw)
+{
+    using S = XShape<M>;
+    constexpr int E = RF<M>::E, T = S::T, RB = S::RB;
+    __shared__ double2 lds[RB * RF<M>::LDS];
+    const int r = threadIdx.x % RB, t = threadIdx.x / RB;  // row fastest: RB-row chunks per kx
+    const int j = xcd_remap(blockIdx.x, gridDim.x) * RB + r;
+    double2* xl = lds + r * RF<M>::LDS;
+    double2 a[E], b[E], nl[E];
+    x_c2r<M>(g0, ny, j, t, xl, tw, rctw, a);      // ux
+    x_c2r<M>(g2, ny, j, t, xl, tw, rctw, b);      // d omega / dx
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = threadIdx.x + m * NT, r = e / M, k = e % M;
-        const double ev = (z[0][m].x * z[2][m].x + z[1][m].x * z[3][m].x) * scale;
-        const double od = (z[0][m].y * z[2][m].y + z[1][m].y * z[3][m].y) * scale;
-        lds[r * P + k] = make_double2(ev, od);
-    }
+    for (int e = 0; e < E; ++e) nl[e] = make_double2(a[e].x * b[e].x, a[e].y * b[e].y);
+    x_c2r<M>(g1, ny, j, t, xl, tw, rctw, a);      // uy
+    x_c2r<M>(g3, ny, j, t, xl, tw, rctw, b);      // d omega / dy
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        nl[e] = make_double2((nl[e].x + a[e].x * b[e].x) * scale, (nl[e].y + a[e].y * b[e].y) * scale);
+    // forward x transform of the real product row (packed even/odd)
+    rf_final_to_first<M>(nl, t, xl);
+    rf_fft<M, -1>(nl, t, xl, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) xl[rf_pad(rf_final_idx<M>(t, e))] = nl[e];
     __syncthreads();
-    lds_fft<M, -1, NT, RB, false>(lds, 1, P, tw);
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = threadIdx.x + m * NT, r = e / M, k = e % M;
-        const double2 zk = lds[r * P + k];
-        const double2 zmk = lds[r * P + (k == 0 ? 0 : M - k)];
+    for (int e = 0; e < E; ++e) {
+        const int k = t + e * T;  // covers 0..M-1, consecutive t -> consecutive kx
+        const double2 zk = xl[rf_pad(k)];
+        const double2 zmk = xl[rf_pad(k == 0 ? 0 : M - k)];
         double2 xm;
-        double2* dst = h + (size_t)(row0 + r) * ld;
-        dst[k] = r2c_unpack(zk, zmk, k, rctw, &xm);
-        if (k == 0) dst[M] = xm;
-        if (k < ld - nh) dst[nh + k] = make_double2(0.0, 0.0);
+        h[(size_t)k * ny + j] = r2c_unpack(zk, zmk, k, rctw, &xm);
+        if (k == 0) h[(size_t)M * ny + j] = xm;
     }
 }
 
@@ -154,32 +182,33 @@ struct YfwdArgs {
     const double* kx;
     const double* ky;
     const int* imin;       // dealias: modes with i >= imin[j] drop the nonlinear term
-    int ld;
+    int nh;
     double nu, dt;
 };
 
 template <int NY, int MODE>
-__global__ __launch_bounds__(ColShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const double2* __restrict__ tw)
+__global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const double2* tw)
 {
-    using S = ColShape<NY>;
-    constexpr int TC = S::TC, NT = S::NT, PER = S::PER;
-    __shared__ double2 lds[TC * NY];
-    const int i0 = blockIdx.x * TC;
+    using S = YShape<NY>;
+    constexpr int E = RF<NY>::E, T = S::T;
+    __shared__ double2 lds[S::CB * RF<NY>::LDS];
+    const int c = threadIdx.x / T, t = threadIdx.x % T;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * S::CB + c;
+    const bool live = i < a.nh;
+    const size_t col = (size_t)(live ? i : 0) * NY;
+    double2 v[E];
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = threadIdx.x + m * NT, j = e / TC, c = e % TC;
-        lds[e] = a.h[(size_t)j * a.ld + i0 + c];
-    }
-    __syncthreads();
-    lds_fft<NY, -1, NT, TC, true>(lds, TC, 1, tw);
+    for (int e = 0; e < E; ++e) v[e] = live ? a.h[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
+    rf_fft<NY, -1>(v, t, lds + c * RF<NY>::LDS, tw);
+    if (!live) return;
+    const double kxv = (double)i;
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = threadIdx.x + m * NT, j = e / TC, c = e % TC;
-        const int i = i0 + c;
-        const size_t q = (size_t)j * a.ld + i;
-        double2 nl = lds[e];
+    for (int e = 0; e < E; ++e) {
+        const int j = rf_final_idx<NY>(t, e);
+        const size_t q = col + j;
+        double2 nl = v[e];
         if (i >= a.imin[j]) nl = make_double2(0.0, 0.0);
-        const double kxv = a.kx[i], kyv = a.ky[j];
+        const double kyv = csh_ky<NY>(j);
         const double vk = a.nu * (kxv * kxv + kyv * kyv);
         const double2 ws = a.ws[q];
         const double2 f = make_double2(-(nl.x + vk * ws.x), -(nl.y + vk * ws.y));
@@ -207,8 +236,8 @@ __global__ __launch_bounds__(ColShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const
 }
 
 // ------------------------------------------------------------ pointwise AB2
-__global__ void k_ab2(const double2* __restrict__ w, double2* __restrict__ wn, const double2* __restrict__ fo,
-                      const double2* __restrict__ fn, double dt, size_t n)
+__global__ void k_ab2(const double2* w, double2* __restrict__ wn, const double2* fo,
+                      const double2* fn, double dt, size_t n)
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         const double2 a = w[q], o = fo[q], f = fn[q];
@@ -220,20 +249,21 @@ __global__ void k_ab2(const double2* __restrict__ w, double2* __restrict__ wn, c
 template <int NY>
 static int launch_yinv(const CshPlan& p, const double2* w, hipStream_t st)
 {
-    using S = ColShape<NY>;
-    hipLaunchKernelGGL(k_csh_yinv<NY>, dim3(p.ld / S::TC), dim3(S::NT), 0, st, w, p.ld, p.kx.as<double>(),
-                       p.ky.as<double>(), p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(),
-                       p.g[3].as<double2>(), p.tw_y);
+    using S = YShape<NY>;
+    hipLaunchKernelGGL(k_csh_yinv<NY>, dim3((p.nh + S::CB - 1) / S::CB), dim3(S::NT), 0, st, w, p.nh,
+                       p.kx.as<double>(), p.ky.as<double>(), p.g[0].as<double2>(), p.g[1].as<double2>(),
+                       p.g[2].as<double2>(), p.g[3].as<double2>(), p.tw_y);
     return TARA_OK;
 }
 
 template <int M>
 static int launch_x(const CshPlan& p, hipStream_t st)
 {
-    using S = RowShape<M>;
+    using S = XShape<M>;
+    TARA_CHECK(p.ny % S::RB == 0, TARA_ERR_UNSUPPORTED, "Ny=%d not a multiple of %d", p.ny, S::RB);
     const double scale = 1.0 / ((double)p.nx * (double)p.ny * (double)p.nx * (double)p.ny);
     hipLaunchKernelGGL(k_csh_x<M>, dim3(p.ny / S::RB), dim3(S::NT), 0, st, p.g[0].as<double2>(),
-                       p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(), p.ld, p.nh, scale,
+                       p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(), p.ny, scale,
                        p.h.as<double2>(), p.tw_x, p.rctw);
     return TARA_OK;
 }
@@ -241,8 +271,8 @@ static int launch_x(const CshPlan& p, hipStream_t st)
 template <int NY>
 static int launch_yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 {
-    using S = ColShape<NY>;
-    dim3 g(p.ld / S::TC), b(S::NT);
+    using S = YShape<NY>;
+    dim3 g((p.nh + S::CB - 1) / S::CB), b(S::NT);
     switch (mode) {
         case CSH_RHS: hipLaunchKernelGGL((k_csh_yfwd<NY, CSH_RHS>), g, b, 0, st, a, p.tw_y); break;
         case CSH_AB2: hipLaunchKernelGGL((k_csh_yfwd<NY, CSH_AB2>), g, b, 0, st, a, p.tw_y); break;
@@ -255,9 +285,8 @@ static int launch_yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_
     return TARA_OK;
 }
 
-#define TARA_SIZE_SWITCH(n, FN, ...)                \
+#define TARA_CSH_SWITCH(n, FN, ...)                 \
     switch (n) {                                    \
-        case 8: return FN<8>(__VA_ARGS__);          \
         case 16: return FN<16>(__VA_ARGS__);        \
         case 32: return FN<32>(__VA_ARGS__);        \
         case 64: return FN<64>(__VA_ARGS__);        \
@@ -270,47 +299,52 @@ static int launch_yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_
         default: set_error("unsupported transform length %d", n); return TARA_ERR_UNSUPPORTED; \
     }
 
-static int yinv(const CshPlan& p, const double2* w, hipStream_t st) { TARA_SIZE_SWITCH(p.ny, launch_yinv, p, w, st) }
-static int xpass(const CshPlan& p, hipStream_t st) { TARA_SIZE_SWITCH(p.nx / 2, launch_x, p, st) }
+static int yinv(const CshPlan& p, const double2* w, hipStream_t st) { TARA_CSH_SWITCH(p.ny, launch_yinv, p, w, st) }
+static int xpass(const CshPlan& p, hipStream_t st) { TARA_CSH_SWITCH(p.nx / 2, launch_x, p, st) }
 static int yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 {
-    TARA_SIZE_SWITCH(p.ny, launch_yfwd, p, mode, a, st)
+    TARA_CSH_SWITCH(p.ny, launch_yfwd, p, mode, a, st)
 }
 
 // ------------------------------------------------------------------ host side
 int csh_plan_init(CshPlan& p, int nx, int ny)
 {
-    TARA_CHECK(is_pow2(nx) && is_pow2(ny) && nx >= 32 && ny >= 32 && nx <= 8192 && ny <= 4096, TARA_ERR_UNSUPPORTED,
-               "derive2csh: grid %dx%d unsupported (powers of two, nx 32..8192, ny 32..4096)", nx, ny);
+    TARA_CHECK(is_pow2(nx) && is_pow2(ny) && nx >= 32 && ny >= 16 && nx <= 8192 && ny <= 4096,
+               TARA_ERR_UNSUPPORTED, "derive2csh: grid %dx%d unsupported (powers of two, nx 32..8192, ny 16..4096)",
+               nx, ny);
     p.nx = nx;
     p.ny = ny;
     p.nh = nx / 2 + 1;
-    p.ld = (p.nh + 7) / 8 * 8;
-    const size_t ns = (size_t)p.ld * ny;
+    const size_t ns = (size_t)p.nh * ny;
     int rc;
     for (int f = 0; f < 4; ++f)
         if ((rc = p.g[f].alloc(ns * sizeof(double2)))) return rc;
     if ((rc = p.h.alloc(ns * sizeof(double2)))) return rc;
-    TARA_HIP(hipMemset(p.h.p, 0, ns * sizeof(double2)));
     // wavenumber and dealias tables, with derive2csh's own expressions
-    std::vector<double> kx(p.ld, 0.0), ky(ny);
+    std::vector<double> kx(p.nh), ky(ny);
     std::vector<int> imin(ny);
     const double pi_s = (double)3.1428f;  // real(8), parameter :: pi = 3.1428 (default-real literal)
     const double L = 2.0 * pi_s;
     for (int i = 0; i < p.nh; ++i) kx[i] = 2.0 * pi_s * (double)i / L;
     for (int j = 0; j < ny; ++j) ky[j] = (j < ny / 2) ? 2.0 * pi_s * (double)j / L : 2.0 * pi_s * ((double)j - ny) / L;
+    for (int i = 0; i < p.nh; ++i)
+        TARA_CHECK(kx[i] == (double)i, TARA_ERR_UNSUPPORTED, "derive2csh kx(%d) is not an integer", i);
+    for (int j = 0; j < ny; ++j)
+        TARA_CHECK(ky[j] == (double)(j < ny / 2 ? j : j - ny), TARA_ERR_UNSUPPORTED, "derive2csh ky(%d) not an integer", j);
     const double cut = ((double)(nx + ny) / 2.0) / 3.0;
     for (int j = 0; j < ny; ++j) {
         int m = p.nh;
         for (int i = 0; i < p.nh; ++i)
-            if (std::sqrt(kx[i] * kx[i] + ky[j] * ky[j]) >= cut) { m = i; break; }
-        // monotone in i (kx increasing): everything from m on is cut
-        imin[j] = m;
+            if (std::sqrt(kx[i] * kx[i] + ky[j] * ky[j]) >= cut) {
+                m = i;
+                break;
+            }
+        imin[j] = m;  // kx increases with i, so every i >= m is cut as well
     }
-    if ((rc = p.kx.alloc(p.ld * sizeof(double)))) return rc;
+    if ((rc = p.kx.alloc(p.nh * sizeof(double)))) return rc;
     if ((rc = p.ky.alloc(ny * sizeof(double)))) return rc;
     if ((rc = p.imin.alloc(ny * sizeof(int)))) return rc;
-    TARA_HIP(hipMemcpy(p.kx.p, kx.data(), p.ld * sizeof(double), hipMemcpyHostToDevice));
+    TARA_HIP(hipMemcpy(p.kx.p, kx.data(), p.nh * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.ky.p, ky.data(), ny * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.imin.p, imin.data(), ny * sizeof(int), hipMemcpyHostToDevice));
     p.tw_y = twiddles(ny);
@@ -339,7 +373,7 @@ int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, double nu, doub
     a.kx = p.kx.as<double>();
     a.ky = p.ky.as<double>();
     a.imin = p.imin.as<int>();
-    a.ld = p.ld;
+    a.nh = p.nh;
     a.nu = nu;
     a.dt = dt;
     if (tm) tm->begin(2, st);

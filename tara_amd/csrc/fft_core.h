@@ -24,10 +24,12 @@
 namespace tara {
 
 // ---------------------------------------------------------------- schedule
-// Radix of the pass that starts with remaining length REM = N / L.
+// Radix of the pass that starts with remaining length REM = N / L: radix 8
+// while it divides, then the remainder (4, 2).  Every kernel of the library
+// uses this one schedule, so one twiddle table per length serves them all.
 __host__ __device__ constexpr int fft_radix(int rem)
 {
-    return rem == 32 ? 8 : (rem % 16 == 0 ? 16 : rem);
+    return rem % 8 == 0 ? 8 : rem;
 }
 // Number of twiddle entries used by all passes before span L (L a product of
 // the schedule's radices).

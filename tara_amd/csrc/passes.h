@@ -9,6 +9,17 @@
 
 namespace tara {
 
+// Workgroups are dealt round-robin over the 8 XCDs (each with its own L2).
+// xcd_remap gives XCD x a contiguous range of logical tiles, so tiles that
+// share cache lines (neighbouring columns or rows) run on the same L2.  It is a
+// bijection on [0, n) for any dealing order; only locality depends on it.
+__device__ __forceinline__ int xcd_remap(int w, int n)
+{
+    const int per = n / 8, rem = n % 8;
+    const int x = w % 8, q = w / 8;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
 // Launch shapes of the column (y / z) passes, by transform length:
 // TC columns per workgroup, NT threads.
 template <int N>
