@@ -13,7 +13,7 @@
 #include <tuple>
 #include <vector>
 
-#include "../../include/tara_hip.h"
+#include "tara_hip.h"
 
 namespace tara {
 

@@ -34,7 +34,8 @@ struct CshPlan {
     DevBuf g[4], h;
     DevBuf kx, ky, imin;
     const double2* tw_y = nullptr;
-    const double2* tw_x = nullptr;
+    const double2* tw_x = nullptr;   // length nx/2 (forward half-length x transform)
+    const double2* tw_xn = nullptr;  // length nx (paired backward x transforms)
     const double2* rctw = nullptr;
 };
 

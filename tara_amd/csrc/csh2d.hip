@@ -2,12 +2,15 @@
 // the right-hand side of src/derive2csh.f95 and the AB2 / RK4 updates
 // (src/ab2csh.f95; classic RK4), as three fused HBM sweeps per RHS.
 //
-// Device layout: every half spectrum is stored ky-major ("transposed"),
+// Device layout: the state half spectra are stored ky-major ("transposed"),
 // element (kx = i, ky = j) at i*Ny + j, so a kx column (one y transform) is a
-// contiguous Ny-element run.  Both y passes then stream whole columns; the x
-// pass, which needs whole rows, reads and writes RB-row chunks (RB*16 bytes
-// contiguous per kx).  The reference's (Nh, Ny) kx-fastest arrays are converted
-// on upload / download (layout.hip).
+// contiguous Ny-element run.  The intermediates G0..G3 and H are stored in
+// 8-row blocks, element (i, j) at ((j/8)*Nh + i)*8 + j%8 (blk8): a y pass reads
+// or writes a column as whole 128-byte lines, and the x pass (RB = 2 rows per
+// workgroup at Nx = 4096) reads 32-byte pieces out of one contiguous
+// 8-row group, whose other pieces belong to the neighbouring workgroups that the
+// XCD-aware mapping runs next to it.  The reference's (Nh, Ny) kx-fastest arrays
+// are converted on upload / download (layout.hip).
 //
 //   pass 1  k_csh_yinv : per kx column read omega_hat, form the four spectra of
 //                        derive2csh.f95:29-71 (i kx omega, i ky omega, i ky psi,
@@ -42,20 +45,11 @@ struct YShape {
 };
 // Rows per workgroup of the x pass: 512 threads where that fits (RB = 4 rows,
 // 64 contiguous bytes per kx, at Nx = 4096), at most 8 rows.
-#ifndef TARA_X_THREADS
-#define TARA_X_THREADS 512
-#endif
-#ifndef TARA_X_WAVES
-#define TARA_X_WAVES 2
-#endif
-template <int M>
-struct XShape {
-    static constexpr int T = RF<M>::T;
-    static constexpr int RB = (TARA_X_THREADS / T) < 2 ? 2 : (TARA_X_THREADS / T) > 8 ? 8 : (TARA_X_THREADS / T);
-    static constexpr int NT = RB * T;
-    static constexpr int MINB = TARA_X_WAVES;  // waves per SIMD the register budget is sized for
-    static_assert(RB * RF<M>::LDS * 16 <= 160 * 1024, "x pass exchange buffer exceeds LDS");
-};
+// Offset of element (kx = i, ky = j) in an 8-row-blocked intermediate.
+__device__ __forceinline__ size_t blk8(int i, int j, int nh)
+{
+    return ((size_t)(j >> 3) * nh + i) * 8 + (j & 7);
+}
 
 // derive2csh's ky: j-1 for j <= Ny/2, j-1-Ny above (exact integers, checked on the host).
 template <int NY>
@@ -110,65 +104,103 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
         double2* o = f == 0 ? g2 : f == 1 ? g3 : f == 2 ? g0 : g1;
         if (live) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) o[col + rf_final_idx<NY>(t, e)] = v[e];
+            for (int e = 0; e < E; ++e) o[blk8(i, rf_final_idx<NY>(t, e), nh)] = v[e];
         }
     }
 }
 
 // --------------------------------------------------------------------- pass 2
-// x c2r of one field for row j: registers end up holding z at rf_final_idx
-// positions m, z[m] = x[2m] + i x[2m+1].
-template <int M>
-__device__ __forceinline__ void x_c2r(const double2* g, int ny, int j, int t, double2* xl,
-                                      const double2* tw, const double2* rctw,
-                                      double2 (&v)[RF<M>::E])
+// Two real c2r transforms in one complex FFT: with X_a, X_b the Hermitian half
+// spectra (kx = 0..M) of two real rows, Y[k] = X_a[k] + i X_b[k] over the full
+// length Nx (X[Nx-k] = conj X[k]) transforms to y = x_a + i x_b, so the product
+// x_a * x_b is Re(y) Im(y).  The imaginary parts of X[0] and X[M] are dropped,
+// as FFTW's c2r does.  Registers end with y at rf_final_idx<NX> positions.
+template <int NX>
+__device__ __forceinline__ void x_c2r_pair(const double2* ga, const double2* gb, int ny, int j, int t,
+                                           double2* xl, const double2* tw, double2 (&v)[RF<NX>::E])
 {
+    constexpr int M = NX / 2;
 #pragma unroll
-    for (int e = 0; e < RF<M>::E; ++e) {
-        const int k = rf_first_idx<M>(t, e);
-        v[e] = c2r_pack(g[(size_t)k * ny + j], g[(size_t)(M - k) * ny + j], k, rctw);
+    for (int e = 0; e < RF<NX>::E; ++e) {
+        const int k = rf_first_idx<NX>(t, e);
+        const int kk = k <= M ? k : NX - k;
+        double2 a = ga[blk8(kk, j, M + 1)], b = gb[blk8(kk, j, M + 1)];
+        if (kk == 0 || kk == M) {
+            a.y = 0.0;
+            b.y = 0.0;
+        }
+        if (k > M) {
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        v[e] = make_double2(a.x - b.y, a.y + b.x);
     }
-    rf_fft<M, +1>(v, t, xl, tw);
+    rf_fft<NX, +1>(v, t, xl, tw);
 }
 
-template <int M>
-__global__ __launch_bounds__(XShape<M>::NT, XShape<M>::MINB) void k_csh_x(const double2* g0,
-                                                          const double2* g1,
-                                                          const double2* g2,
-                                                          const double2* g3, int ny, double scale,
-                                                          double2* h, const double2* tw,
-                                                          const double2* rctw)
+template <int NX>
+struct XShape {
+    static constexpr int M = NX / 2;
+    static constexpr int T = RF<NX>::T;    // threads per row (backward transforms, length NX)
+    static constexpr int TM = RF<M>::T;    // of which active in the forward half-length transform
+    static constexpr int RB = (1024 / T) < 1 ? 1 : (1024 / T) > 8 ? 8 : (1024 / T);
+    static constexpr int NT = RB * T;
+    // Per-row exchange buffer (complex elements).  Rows are interleaved across
+    // lanes, so successive row buffers are offset by 4 elements (64 B) to land
+    // on different LDS banks.
+    static constexpr int LDSR = RF<NX>::LDS + 4;
+    static_assert(RB * LDSR * 16 <= 160 * 1024, "x pass exchange buffer exceeds LDS");
+    static_assert(2 * TM == T, "forward transform uses half of the row's threads");
+};
+
+template <int NX>
+__global__ __launch_bounds__(XShape<NX>::NT) void k_csh_x(const double2* g0, const double2* g1, const double2* g2,
+                                                           const double2* g3, int ny, double scale,
+                                                           double2* __restrict__ h, const double2* tw_n,
+                                                           const double2* tw_m, const double2* rctw)
 {
-    using S = XShape<M>;
-    constexpr int E = RF<M>::E, T = S::T, RB = S::RB;
-    __shared__ double2 lds[RB * RF<M>::LDS];
+    using S = XShape<NX>;
+    constexpr int E = RF<NX>::E, T = S::T, M = S::M, RB = S::RB;
+    __shared__ double2 lds[RB * S::LDSR];
     const int r = threadIdx.x % RB, t = threadIdx.x / RB;  // row fastest: RB-row chunks per kx
     const int j = xcd_remap(blockIdx.x, gridDim.x) * RB + r;
-    double2* xl = lds + r * RF<M>::LDS;
-    double2 a[E], b[E], nl[E];
-    x_c2r<M>(g0, ny, j, t, xl, tw, rctw, a);      // ux
-    x_c2r<M>(g2, ny, j, t, xl, tw, rctw, b);      // d omega / dx
+    double2* xl = lds + r * S::LDSR;
+    double nl[E];
+    double2 v[E];
+    x_c2r_pair<NX>(g0, g2, ny, j, t, xl, tw_n, v);   // ux + i d omega/dx
 #pragma unroll
-    for (int e = 0; e < E; ++e) nl[e] = make_double2(a[e].x * b[e].x, a[e].y * b[e].y);
-    x_c2r<M>(g1, ny, j, t, xl, tw, rctw, a);      // uy
-    x_c2r<M>(g3, ny, j, t, xl, tw, rctw, b);      // d omega / dy
+    for (int e = 0; e < E; ++e) nl[e] = v[e].x * v[e].y;
+    x_c2r_pair<NX>(g1, g3, ny, j, t, xl, tw_n, v);   // uy + i d omega/dy
 #pragma unroll
-    for (int e = 0; e < E; ++e)
-        nl[e] = make_double2((nl[e].x + a[e].x * b[e].x) * scale, (nl[e].y + a[e].y * b[e].y) * scale);
-    // forward x transform of the real product row (packed even/odd)
-    rf_final_to_first<M>(nl, t, xl);
-    rf_fft<M, -1>(nl, t, xl, tw);
+    for (int e = 0; e < E; ++e) nl[e] = (nl[e] + v[e].x * v[e].y) * scale;
+    // forward x transform of the real product row: pack z[m] = nl[2m] + i nl[2m+1]
+    // for the half-length FFT (threads t < TM carry it; the others run the same
+    // code on the unused upper half of the row buffer so every barrier matches).
+    double* xr = reinterpret_cast<double*>(xl);
 #pragma unroll
-    for (int e = 0; e < E; ++e) xl[rf_pad(rf_final_idx<M>(t, e))] = nl[e];
+    for (int e = 0; e < E; ++e) xr[rf_final_idx<NX>(t, e)] = nl[e];
     __syncthreads();
+    double2 z[RF<M>::E];
+    const int tm = t % S::TM;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int k = t + e * T;  // covers 0..M-1, consecutive t -> consecutive kx
+    for (int e = 0; e < RF<M>::E; ++e) {
+        const int m = rf_first_idx<M>(tm, e);
+        z[e] = make_double2(xr[2 * m], xr[2 * m + 1]);
+    }
+    __syncthreads();
+    double2* zl = xl + (t < S::TM ? 0 : RF<M>::LDS);
+    rf_fft<M, -1>(z, tm, zl, tw_m);
+    if (t < S::TM) {
+#pragma unroll
+        for (int e = 0; e < RF<M>::E; ++e) xl[rf_pad(rf_final_idx<M>(tm, e))] = z[e];
+    }
+    __syncthreads();
+    for (int k = t; k < M; k += T) {  // consecutive t -> consecutive kx
         const double2 zk = xl[rf_pad(k)];
         const double2 zmk = xl[rf_pad(k == 0 ? 0 : M - k)];
         double2 xm;
-        h[(size_t)k * ny + j] = r2c_unpack(zk, zmk, k, rctw, &xm);
-        if (k == 0) h[(size_t)M * ny + j] = xm;
+        h[blk8(k, j, M + 1)] = r2c_unpack(zk, zmk, k, rctw, &xm);
+        if (k == 0) h[blk8(M, j, M + 1)] = xm;
     }
 }
 
@@ -198,7 +230,7 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const d
     const size_t col = (size_t)(live ? i : 0) * NY;
     double2 v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = live ? a.h[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
+    for (int e = 0; e < E; ++e) v[e] = live ? a.h[blk8(i, rf_first_idx<NY>(t, e), a.nh)] : make_double2(0.0, 0.0);
     rf_fft<NY, -1>(v, t, lds + c * RF<NY>::LDS, tw);
     if (!live) return;
     const double kxv = (double)i;
@@ -256,15 +288,15 @@ static int launch_yinv(const CshPlan& p, const double2* w, hipStream_t st)
     return TARA_OK;
 }
 
-template <int M>
+template <int NX>
 static int launch_x(const CshPlan& p, hipStream_t st)
 {
-    using S = XShape<M>;
+    using S = XShape<NX>;
     TARA_CHECK(p.ny % S::RB == 0, TARA_ERR_UNSUPPORTED, "Ny=%d not a multiple of %d", p.ny, S::RB);
     const double scale = 1.0 / ((double)p.nx * (double)p.ny * (double)p.nx * (double)p.ny);
-    hipLaunchKernelGGL(k_csh_x<M>, dim3(p.ny / S::RB), dim3(S::NT), 0, st, p.g[0].as<double2>(),
+    hipLaunchKernelGGL(k_csh_x<NX>, dim3(p.ny / S::RB), dim3(S::NT), 0, st, p.g[0].as<double2>(),
                        p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(), p.ny, scale,
-                       p.h.as<double2>(), p.tw_x, p.rctw);
+                       p.h.as<double2>(), p.tw_xn, p.tw_x, p.rctw);
     return TARA_OK;
 }
 
@@ -300,7 +332,7 @@ static int launch_yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_
     }
 
 static int yinv(const CshPlan& p, const double2* w, hipStream_t st) { TARA_CSH_SWITCH(p.ny, launch_yinv, p, w, st) }
-static int xpass(const CshPlan& p, hipStream_t st) { TARA_CSH_SWITCH(p.nx / 2, launch_x, p, st) }
+static int xpass(const CshPlan& p, hipStream_t st) { TARA_CSH_SWITCH(p.nx, launch_x, p, st) }
 static int yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 {
     TARA_CSH_SWITCH(p.ny, launch_yfwd, p, mode, a, st)
@@ -309,8 +341,8 @@ static int yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 // ------------------------------------------------------------------ host side
 int csh_plan_init(CshPlan& p, int nx, int ny)
 {
-    TARA_CHECK(is_pow2(nx) && is_pow2(ny) && nx >= 32 && ny >= 16 && nx <= 8192 && ny <= 4096,
-               TARA_ERR_UNSUPPORTED, "derive2csh: grid %dx%d unsupported (powers of two, nx 32..8192, ny 16..4096)",
+    TARA_CHECK(is_pow2(nx) && is_pow2(ny) && nx >= 32 && ny >= 16 && nx <= 4096 && ny <= 4096,
+               TARA_ERR_UNSUPPORTED, "derive2csh: grid %dx%d unsupported (powers of two, 32..4096 x 16..4096)",
                nx, ny);
     p.nx = nx;
     p.ny = ny;
@@ -349,8 +381,9 @@ int csh_plan_init(CshPlan& p, int nx, int ny)
     TARA_HIP(hipMemcpy(p.imin.p, imin.data(), ny * sizeof(int), hipMemcpyHostToDevice));
     p.tw_y = twiddles(ny);
     p.tw_x = twiddles(nx / 2);
+    p.tw_xn = twiddles(nx);
     p.rctw = rc_twiddles(nx);
-    TARA_CHECK(p.tw_y && p.tw_x && p.rctw, TARA_ERR_ALLOC, "twiddle table allocation failed");
+    TARA_CHECK(p.tw_y && p.tw_x && p.tw_xn && p.rctw, TARA_ERR_ALLOC, "twiddle table allocation failed");
     return TARA_OK;
 }
 

@@ -11,8 +11,10 @@
 // registers, so a transform costs (passes - 1) LDS round trips and no LDS
 // staging of the global traffic.
 //
-// LDS positions are padded by one element per 16 (rf_pad) so strided scatters
-// (stride 8 or 16 elements = 128 or 256 B) spread over the banks.
+// LDS positions are padded by one element per 8 (rf_pad): the LDS serves 8 lanes
+// of 16-byte accesses per cycle (128 B), and with this padding the radix-8
+// scatter (stride 8 elements) and every gather of the schedule put 8
+// consecutive lanes on 8 distinct 16-byte bank groups.
 // Radix schedule and twiddle tables are those of fft_core.h.
 #pragma once
 #include "fft_core.h"
@@ -23,10 +25,10 @@ template <int N>
 struct RF {
     static constexpr int E = N < 8 ? N : 8;  // elements per thread
     static constexpr int T = N / E;            // threads per sequence
-    static constexpr int LDS = N + N / 16;     // padded exchange length
+    static constexpr int LDS = N + N / 8;      // padded exchange length
 };
 
-__device__ __forceinline__ int rf_pad(int n) { return n + (n >> 4); }
+__device__ __forceinline__ int rf_pad(int n) { return n + (n >> 3); }
 
 // Span of the last pass.
 __host__ __device__ constexpr int rf_last_span(int n)
