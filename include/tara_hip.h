@@ -79,6 +79,49 @@ int tara_csh_destroy(tara_csh* ctx);
 int tara_csh_set_timing(tara_csh* ctx, int on);
 int tara_csh_kernel_stats(tara_csh* ctx, int nclass, long* counts, double* ms);
 
+/* ------------------------------------ 3D compressible isothermal MHD (3com)
+ * Fields, in the order of the reference's argument lists: rho, rho_ux, rho_uy,
+ * rho_uz, E, Bx, By, Bz; each a (Nh, Ny, Nz) half spectrum.  Physical box
+ * lengths are the values after the reference multiplies the input by pi
+ * (src/main3com.f95:203); mu and eta are the viscosity and resistivity of the
+ * diffusion terms (the reference hard-wires both to 1/450,
+ * src/derive3com.f95:1023-1037); cs is the isothermal sound speed (P = cs^2 rho,
+ * src/derive3com.f95:395).  Grids: powers of two, 16..256 per axis.
+ */
+typedef struct {
+    double lx, ly, lz;
+    double mu, eta;
+    double cs;
+    double dt;
+} tara_mhd_params;
+
+/* Replaces src/derive3com.f95:5 derive3com(Nx,Ny,Nz,Nh,pi,time, 8 state arrays,
+ * 8 interleaved (d_*_dt_old, d_*_dt_new) pairs).  rhs[f] receives d_*_dt_new.
+ * The energy right-hand side rhs[4] is returned as zero: ab3com never reads it
+ * (the energy is frozen, src/ab3com.f95:51). */
+int tara_derive3com(int nx, int ny, int nz, int nh, const tara_mhd_params* p, const double* const state[8],
+                    double* const rhs[8]);
+
+/* Replaces src/ab3com.f95:6 ab3com: new = old + (3/2 f_n - 1/2 f_{n-1}) dt for
+ * rho, rho u, B; E is copied unchanged. */
+int tara_ab3com(int nx, int ny, int nz, int nh, double dt, const double* const state[8],
+                double* const state_new[8], const double* const f_old[8], const double* const f_new[8]);
+
+/* Device-resident AB2 stepping of the same equations (the derive3com + ab3com
+ * + rotation of src/main3com.f95:536-617, without the diagnostics).  The AB2
+ * history starts at zero on upload. */
+typedef struct tara_mhd tara_mhd;
+int tara_mhd_create(int nx, int ny, int nz, const tara_mhd_params* p, tara_mhd** out);
+int tara_mhd_upload(tara_mhd* ctx, const double* const state[8]);
+int tara_mhd_download(tara_mhd* ctx, double* const state[8]);
+int tara_mhd_advance(tara_mhd* ctx, int nsteps);
+int tara_mhd_sync(tara_mhd* ctx);
+int tara_mhd_destroy(tara_mhd* ctx);
+/* Kernel timing as for tara_csh: classes 0 = inverse z+y passes, 1 = x pass,
+ * 2 = forward y pass, 3 = forward z pass + assembly + update. */
+int tara_mhd_set_timing(tara_mhd* ctx, int on);
+int tara_mhd_kernel_stats(tara_mhd* ctx, int nclass, long* counts, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

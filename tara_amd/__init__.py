@@ -6,6 +6,8 @@ array layout follow the Fortran routines they replace:
 
   derive2csh(Nx, Ny, Nh, nu, omegak)            src/derive2csh.f95:3
   ab2csh(Nx, Ny, Nh, dt, omegak, f_old, f_new)  src/ab2csh.f95:3
+  derive3com(Nx, Ny, Nz, Nh, params, state)     src/derive3com.f95:5
+  ab3com(Nx, Ny, Nz, Nh, dt, state, f_old, f_new)  src/ab3com.f95:6
   fft_r2c / fft_c2r                             dfftw_plan_dft_{r2c,c2r}_{2d,3d} + execute
 
 Spectral arrays are complex128 half spectra (Nh, Ny[, Nz]) in Fortran order
@@ -30,6 +32,12 @@ class TaraError(RuntimeError):
     pass
 
 
+class MhdParams(C.Structure):
+    """tara_mhd_params: physical box (input.ini value times pi), diffusion
+    coefficients, isothermal sound speed and time step."""
+    _fields_ = [(n, C.c_double) for n in ("lx", "ly", "lz", "mu", "eta", "cs", "dt")]
+
+
 def lib():
     """Loads libtara_hip.so (built by ``python -m tara_amd.build``)."""
     global _lib
@@ -52,6 +60,17 @@ def lib():
         L.tara_csh_destroy.argtypes = [C.c_void_p]
         L.tara_csh_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.tara_csh_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
+        P8 = C.c_void_p * 8
+        L.tara_derive3com.argtypes = [C.c_int] * 4 + [C.POINTER(MhdParams), P8, P8]
+        L.tara_ab3com.argtypes = [C.c_int] * 4 + [C.c_double, P8, P8, P8, P8]
+        L.tara_mhd_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(MhdParams), C.POINTER(C.c_void_p)]
+        L.tara_mhd_upload.argtypes = [C.c_void_p, P8]
+        L.tara_mhd_download.argtypes = [C.c_void_p, P8]
+        L.tara_mhd_advance.argtypes = [C.c_void_p, C.c_int]
+        for n in ("tara_mhd_sync", "tara_mhd_destroy"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.tara_mhd_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.tara_mhd_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
         _lib = L
     return _lib
 
@@ -157,6 +176,74 @@ class CshSolver:
     def close(self):
         if self._h:
             lib().tara_csh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------- 3D compressible MHD (3com)
+def _p8(arrs):
+    return (C.c_void_p * 8)(*[a.ctypes.data for a in arrs])
+
+
+def derive3com(Nx, Ny, Nz, Nh, params, state):
+    """8 RHS spectra d_*_dt_new = derive3com(...) (src/derive3com.f95:5);
+    state = [rho, rho_ux, rho_uy, rho_uz, E, Bx, By, Bz], each (Nh, Ny, Nz)."""
+    st = [_spec(a) for a in state]
+    out = [np.zeros_like(st[0], order="F") for _ in range(8)]
+    _check(lib().tara_derive3com(Nx, Ny, Nz, Nh, C.byref(params), _p8(st), _p8(out)))
+    return out
+
+
+def ab3com(Nx, Ny, Nz, Nh, dt, state, f_old, f_new):
+    """state_new = ab3com(...) (src/ab3com.f95:6); E is copied unchanged."""
+    st, fo, fn = ([_spec(a) for a in x] for x in (state, f_old, f_new))
+    out = [np.zeros_like(st[0], order="F") for _ in range(8)]
+    _check(lib().tara_ab3com(Nx, Ny, Nz, Nh, dt, _p8(st), _p8(out), _p8(fo), _p8(fn)))
+    return out
+
+
+class MhdSolver:
+    """Device-resident AB2 stepping of derive3com + ab3com (main3com.f95:536-617)."""
+
+    def __init__(self, nx, ny, nz, params):
+        self.shape = (nx // 2 + 1, ny, nz)
+        h = C.c_void_p()
+        _check(lib().tara_mhd_create(nx, ny, nz, C.byref(params), C.byref(h)))
+        self._h = h
+
+    def upload(self, state):
+        st = [_spec(a) for a in state]
+        assert all(a.shape == self.shape for a in st)
+        _check(lib().tara_mhd_upload(self._h, _p8(st)))
+
+    def download(self):
+        out = [np.zeros(self.shape, np.complex128, order="F") for _ in range(8)]
+        _check(lib().tara_mhd_download(self._h, _p8(out)))
+        return out
+
+    def advance(self, nsteps):
+        _check(lib().tara_mhd_advance(self._h, nsteps))
+
+    def sync(self):
+        _check(lib().tara_mhd_sync(self._h))
+
+    def set_timing(self, on):
+        _check(lib().tara_mhd_set_timing(self._h, 1 if on else 0))
+
+    def kernel_stats(self, nclass=4):
+        counts = (C.c_long * nclass)()
+        ms = (C.c_double * nclass)()
+        _check(lib().tara_mhd_kernel_stats(self._h, nclass, counts, ms))
+        return list(counts), list(ms)
+
+    def close(self):
+        if self._h:
+            lib().tara_mhd_destroy(self._h)
             self._h = None
 
     def __del__(self):

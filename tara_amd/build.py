@@ -53,17 +53,16 @@ def build_hip(jobs=None):
 
 
 def build_fortran():
-    """ISO_C_BINDING wrappers keeping the reference's subroutine names."""
+    """ISO_C_BINDING wrappers keeping the reference's subroutine names
+    (libtara_fortran.so), and the Fortran test drivers under tests/fortran/."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
-    fdir = os.path.join(HERE, "fortran")
-    srcs = sorted(f for f in os.listdir(fdir) if f.endswith(".f90"))
-    if not srcs or not os.path.exists(fc):
+    if not os.path.exists(fc):
         return None
+    fdir = os.path.join(HERE, "fortran")
     os.makedirs(OBJ, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(fdir) if f.endswith(".f90"))
+    order = sorted(srcs, key=lambda s: (0 if s.endswith("_mod.f90") else 1, s))  # modules first
     objs = []
-    for s in srcs:  # module first (sorted: tara_hip_mod.f90 is compiled before users via order below)
-        pass
-    order = sorted(srcs, key=lambda s: (0 if s.endswith("_mod.f90") else 1, s))
     for s in order:
         o = os.path.join(OBJ, s[:-4] + ".fo")
         _run([fc, "-O2", "-fPIC", "-J" + OBJ, "-c", os.path.join(fdir, s), "-o", o])
@@ -71,6 +70,12 @@ def build_fortran():
     tmp = FLIB + ".tmp"
     _run([fc, "-shared", "-o", tmp] + objs + ["-L" + HERE, "-ltara_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FLIB)
+    tdir = os.path.join(ROOT, "tests", "fortran")
+    for s in sorted(f for f in os.listdir(tdir) if f.endswith(".f90")):
+        exe = os.path.join(tdir, s[:-4])
+        _run([fc, "-O2", "-J" + OBJ, os.path.join(tdir, s), "-o", exe + ".tmp", "-L" + HERE,
+              "-ltara_fortran", "-ltara_hip", "-Wl,-rpath,$ORIGIN/../../tara_amd"])
+        os.replace(exe + ".tmp", exe)
     return FLIB
 
 

@@ -61,13 +61,14 @@ struct DevBuf {
 };
 
 // Per-device immutable tables, built once and kept for the process lifetime.
-//   twiddles(n)   : Stockham per-pass table for a length-n complex FFT (fft_core.h)
-//   rc_twiddles(n): e^{-2 pi i k/n}, k = 0..n/2 (real <-> half-length complex packing)
-const double2* twiddles(int n);
+//   twiddles(n, rmax): Stockham per-pass table for a length-n complex FFT whose
+//                      schedule caps the radix at rmax (8 or 4; fft_core.h)
+//   rc_twiddles(n)   : e^{-2 pi i k/n}, k = 0..n/2 (real <-> half-length complex packing)
+const double2* twiddles(int n, int rmax = 8);
 const double2* rc_twiddles(int n);
 
-// Host-side table generators (also used by tests through the C ABI).
-void host_twiddles(int n, std::vector<double2>& out);
+// Host-side table generators.
+void host_twiddles(int n, int rmax, std::vector<double2>& out);
 void host_rc_twiddles(int n, std::vector<double2>& out);
 
 int current_device();

@@ -68,12 +68,12 @@ static double2 unit_root(long num, long den)
     return make_double2((double)c, (double)(-s));
 }
 
-void host_twiddles(int n, std::vector<double2>& out)
+void host_twiddles(int n, int rmax, std::vector<double2>& out)
 {
-    out.assign(std::max(1, fft_tw_size(n)), make_double2(0.0, 0.0));
+    out.assign(std::max(1, fft_tw_size(n, rmax)), make_double2(0.0, 0.0));
     int span = 1, off = 0;
     while (span < n) {
-        int r = fft_radix(n / span);
+        int r = fft_radix(n / span, rmax);
         for (int k = 0; k < span; ++k)
             for (int s = 1; s < r; ++s) out[off + k * (r - 1) + (s - 1)] = unit_root((long)k * s, (long)span * r);
         off += span * (r - 1);
@@ -91,6 +91,8 @@ namespace {
 std::mutex g_tab_mu;
 std::map<std::tuple<int, int, int>, void*> g_tabs;  // (device, kind, n) -> device ptr
 
+// kind: 0 = Stockham table with radix cap 8, 1 = real<->complex packing roots,
+// 4 = Stockham table with radix cap 4.
 const double2* cached_table(int kind, int n)
 {
     int dev = current_device();
@@ -99,10 +101,10 @@ const double2* cached_table(int kind, int n)
     auto it = g_tabs.find(key);
     if (it != g_tabs.end()) return static_cast<const double2*>(it->second);
     std::vector<double2> h;
-    if (kind == 0)
-        host_twiddles(n, h);
-    else
+    if (kind == 1)
         host_rc_twiddles(n, h);
+    else
+        host_twiddles(n, kind == 4 ? 4 : 8, h);
     void* d = nullptr;
     if (hipMalloc(&d, h.size() * sizeof(double2)) != hipSuccess) return nullptr;
     if (hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
@@ -114,7 +116,7 @@ const double2* cached_table(int kind, int n)
 }
 }  // namespace
 
-const double2* twiddles(int n) { return cached_table(0, n); }
+const double2* twiddles(int n, int rmax) { return cached_table(rmax == 4 ? 4 : 0, n); }
 const double2* rc_twiddles(int n) { return cached_table(1, n); }
 
 }  // namespace tara

@@ -24,26 +24,26 @@
 namespace tara {
 
 // ---------------------------------------------------------------- schedule
-// Radix of the pass that starts with remaining length REM = N / L: radix 8
-// while it divides, then the remainder (4, 2).  Every kernel of the library
-// uses this one schedule, so one twiddle table per length serves them all.
-__host__ __device__ constexpr int fft_radix(int rem)
+// Radix of the pass that starts with remaining length REM = N / L, for a
+// schedule capped at radix RMAX (8 or 4): RMAX while it divides, then the
+// remainder.  A twiddle table belongs to one (N, RMAX) schedule.
+__host__ __device__ constexpr int fft_radix(int rem, int rmax = 8)
 {
-    return rem % 8 == 0 ? 8 : rem;
+    return rem % rmax == 0 ? rmax : rem;
 }
 // Number of twiddle entries used by all passes before span L (L a product of
 // the schedule's radices).
-__host__ __device__ constexpr int fft_tw_offset(int n, int l)
+__host__ __device__ constexpr int fft_tw_offset(int n, int l, int rmax = 8)
 {
     int off = 0, span = 1;
     while (span < l) {
-        int r = fft_radix(n / span);
+        int r = fft_radix(n / span, rmax);
         off += span * (r - 1);
         span *= r;
     }
     return off;
 }
-__host__ __device__ constexpr int fft_tw_size(int n) { return fft_tw_offset(n, n); }
+__host__ __device__ constexpr int fft_tw_size(int n, int rmax = 8) { return fft_tw_offset(n, n, rmax); }
 
 // --------------------------------------------------------------- complex ops
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
