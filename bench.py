@@ -24,6 +24,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and, in a
+# separate run, WRITE_SIZE over this script (--steps 2 --warmup 1
+# --no-cpu-baseline), each scaled by what the counter reads for a device copy
+# of known size (tara_amd/calib/copy_probe: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+MEASURED_TRAFFIC = {
+    ("k_csh_yinv", 4096): 6.7237e8,
+    ("k_csh_x", 4096): 7.2427e8,
+    ("k_csh_yfwd", 4096): 6.7437e8,      # mean over the four RK4 stage variants
+    ("k_mhd_col_inverse_zy", 256): 3.9932e9,   # two launches (z, then y), 1.9966e9 each
+    ("k_mhd_x", 256): 2.9069e9,
+    ("k_mhd_col_forward", 256): 3.9931e9,
+    ("k_mhd_zfwd", 256): 6.4691e9,
+}
 METRIC = "grid-pt steps/s FP64 (2D 4096², 3D 256³ MHD); FNO sim-time/s at ≤5% L2 err"
 
 
@@ -46,6 +60,47 @@ def cpu_baseline(n):
                          else ", C restatement (not FFTW)")}
 
 
+def bench_mhd(steps, warmup, n=256):
+    """Secondary line: 3D compressible MHD, AB2, 256^3, nu = eta = 1e-3,
+    dt = 5e-4 (derive3com + ab3com on the GPU, state resident)."""
+    import tara_amd as T
+    from tara_amd.ic import random_phase_mhd
+
+    L = 2.0 * 3.14159265358979323846
+    prm = T.MhdParams(L, L, L, 1e-3, 1e-3, 10.0, 5e-4)
+    s = T.MhdSolver(n, n, n, prm)
+    s.upload(random_phase_mhd(n, seed=1))
+    s.advance(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.advance(steps)
+    s.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    s.set_timing(True)
+    s.advance(steps)
+    s.sync()
+    counts, kms = s.kernel_stats(4)
+    s.close()
+    S3 = 16.0 * (n // 2 + 1) * n * n
+    # algorithmic bytes per timed region: inverse z and y passes (two launches)
+    # 7 fields in + 7 out each;
+    # x pass 7 in, 14 out; forward y 14 + 14; forward z reads 14 fluxes + 7 state
+    # + 7 history and writes 7 state + 7 history.  119 S3 per step.
+    names = ["k_mhd_col_inverse_zy", "k_mhd_x", "k_mhd_col_forward", "k_mhd_zfwd"]
+    model = [28 * S3, 21 * S3, 28 * S3, 42 * S3]
+    avg = {c: kms[c] / counts[c] for c in range(4) if counts[c]}
+    dom = max(avg, key=lambda c: kms[c])
+    achieved = model[dom] / (avg[dom] * 1e-3) / 1e9
+    return {"workload": f"3D compressible isothermal MHD, AB2, {n}^3, FP64, nu = eta = 1e-3, dt = 5e-4",
+            "value": n ** 3 / (ms * 1e-3), "unit": "grid-pt steps/s", "ms_per_step": ms,
+            "step_model_bytes": 119 * S3, "step_model_GBps": round(119 * S3 / (ms * 1e-3) / 1e9, 1),
+            "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": MEASURED_TRAFFIC.get((names[dom], n))},
+            "kernels": {names[c]: {"avg_ms": round(avg[c], 4), "launches": counts[c],
+                                   "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +108,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-3d", action="store_true", help="skip the secondary 3D MHD measurement")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -89,13 +145,17 @@ def main():
     finite = bool(abs(w).max() < 1e300)
 
     S = 16.0 * (n // 2 + 1) * n                      # one half spectrum, bytes
-    model = {0: 5 * S, 1: 5 * S, 2: 5 * S}           # algorithmic bytes per launch
+    # algorithmic bytes per launch: inverse y reads 1 field and writes 4; the x
+    # pass reads 4 and writes 1; the forward y pass (RK4 stages 1..4) moves
+    # 4, 6, 6, 4 fields, 5 on average.  60 S per RK4 step.
+    model = {0: 5 * S, 1: 5 * S, 2: 5 * S}
     names = {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd"}
     avg = {c: ms[c] / counts[c] for c in range(3) if counts[c]}
     dom = max(avg, key=lambda c: avg[c] * counts[c])
     achieved = model[dom] / (avg[dom] * 1e-3) / 1e9
     kernels = {names[c]: {"avg_ms": round(avg[c], 4), "launches": counts[c],
                           "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}
+    mhd = None if a.no_3d else bench_mhd(a.steps, a.warmup)
     out = {
         "metric": METRIC,
         "value": n * n / (ms_per_step * 1e-3),
@@ -113,13 +173,15 @@ def main():
                    "nx": n, "ny": n, "nu": nu, "dt": dt, "scheme": "rk4", "dealias": "circular 2/3",
                    "ic": "random-phase E(k)~k^4 exp(-(k/4)^2), unit KE, seed 1"},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((names[dom], n)),
                      "algorithmic_bytes_per_launch": model[dom]},
         "kernels": kernels,
         "step_model_bytes": 60 * S,
         "step_model_GBps": round(60 * S / (ms_per_step * 1e-3) / 1e9, 1),
         "state_finite": finite,
         "cpu_baseline": cpu,
+        "secondary_3d_mhd": mhd,
     }
     print(json.dumps(out))
 

@@ -49,6 +49,9 @@ def build_hip(jobs=None):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    probe = os.path.join(HERE, "calib", "copy_probe")
+    _run([HIPCC] + HIP_FLAGS + [probe + ".hip", "-o", probe + ".tmp"])
+    os.replace(probe + ".tmp", probe)
     return LIB
 
 
