@@ -79,6 +79,107 @@ int tara_csh_destroy(tara_csh* ctx);
 int tara_csh_set_timing(tara_csh* ctx, int on);
 int tara_csh_kernel_stats(tara_csh* ctx, int nclass, long* counts, double* ms);
 
+/* ------------------------------------- 2D screened vorticity (css family)
+ * Replaces src/derive2css.f95:6 derive2css(Nx,Ny,Nh,alpha,nu,time,omegak,
+ * omegak_new,dt_omegak_old,dt_omegak_new): psi = omega/(k^2 + lambda) with
+ * psi(0,0) = 0, omega_d = k^2 psi, f = -(alpha mask(u . grad omega_d) +
+ * nu k^2 omega_d).  The reference leaves Lx, Ly and lambda uninitialised
+ * (src/derive2css.f95:11); this takes Lx = Ly = 2 pi as derive2csh does
+ * (integer wavenumbers) and lambda as an argument (src/main2css.f95:87 sets 10).
+ * lambda = 0, alpha = 1 gives derive2csh's right-hand side.
+ */
+int tara_derive2css(int nx, int ny, int nh, double alpha, double nu, double lambda, const double* omegak,
+                    double* dt_omegak_new);
+/* Replaces src/ab2css.f95:6 (the same AB2 update as ab2csh). */
+int tara_ab2css(int nx, int ny, int nh, double dt, const double* omegak, double* omegak_new,
+                const double* dt_omegak_old, const double* dt_omegak_new);
+/* Replaces src/rk42css.f95:6 rk42css with the classic RK4 step it intends
+ * (stages at dt/2, dt/2, dt; the reference reads an uninitialised slope,
+ * src/rk42css.f95:15-16, and never uses its derive2css results). */
+int tara_rk42css(int nx, int ny, int nh, double alpha, double nu, double lambda, double dt, const double* omegak,
+                 double* omegak_new);
+/* Device-resident css stepping; the context is driven by tara_csh_upload /
+ * advance / download / sync / destroy. */
+int tara_css_create(int nx, int ny, double alpha, double nu, double lambda, double dt, int scheme,
+                    tara_csh** out);
+
+/* ------------------------- 2D compressible families (coh hydro, com MHD)
+ * Box and wavenumbers as the reference forms them: kx = 2 pi (i-1)/Lx with the
+ * caller's pi (an argument of derive2coh / derive2com) and Lx, Ly already
+ * multiplied by pi (src/main2coh.f95:101).  Circular dealias
+ * sqrt(kx^2+ky^2) >= (Nx+Ny)/2/3 on every term.  Grids: powers of two,
+ * 16..256 per axis.
+ */
+typedef struct {
+    double pi;       /* the pi argument */
+    double lx, ly;   /* COMMON /comm/ Lx, Ly (src/main2coh.f95:29) */
+    double mux, muy; /* viscosities of the x and y second derivatives */
+    double cs;       /* isothermal sound speed */
+    double dt;       /* resident stepping only */
+} tara_coh_params;
+
+/* Replaces src/derive2coh.f95:5 derive2coh(Nx,Ny,Nh,pi,time,nk,ukx,uky,
+ * dn_dt_old,dux_dt_old,duy_dt_old,dn_dt_new,dux_dt_new,duy_dt_new):
+ * state = {nk, ukx, uky} (in), rhs = {dn_dt_new, dux_dt_new, duy_dt_new} (out).
+ * The _old arrays are unused by the reference and not passed. */
+int tara_derive2coh(int nx, int ny, int nh, const tara_coh_params* p, const double* const state[3],
+                    double* const rhs[3]);
+/* Replaces src/ab2coh.f95:5: new = old + (3/2 f_n - 1/2 f_{n-1}) dt per field. */
+int tara_ab2coh(int nx, int ny, int nh, double dt, const double* const state[3], double* const state_new[3],
+                const double* const f_old[3], const double* const f_new[3]);
+/* Replaces src/euler2coh.f95:5: new = old + slope dt per field. */
+int tara_euler2coh(int nx, int ny, int nh, double dt, const double* const state[3], double* const state_new[3],
+                   const double* const slope[3]);
+/* Replaces rk4_nk2coh / rk4_ukx2coh / rk4_uky2coh (src/rk4nk2coh.f95:6 and
+ * its twins) on the field each one advances.  Their derive2coh calls write
+ * local arrays that are never read, so all four stage slopes are the input
+ * slope: new = old + dt/6 (((s + 2s) + 2s) + s). */
+int tara_rk4coh_field(int nx, int ny, int nh, double dt, const double* field, double* field_new,
+                      const double* slope);
+/* Replaces pc_nk2coh / pc_ukx2coh / pc_uky2coh (src/pcnk2coh.f95:6): predictor
+ * and corrector slopes are both the input slope: new = old + ((s + s) dt)/2. */
+int tara_pccoh_field(int nx, int ny, int nh, double dt, const double* field, double* field_new,
+                     const double* slope);
+
+typedef struct {
+    double pi;
+    double lx, ly;   /* COMMON /comm/ Lx, Ly (src/main2com.f95:43) */
+    double mu, eta;  /* viscosity, resistivity */
+    double cs;       /* P = CS^2 rho */
+    double kf, a;    /* forcing F = rho A (-sin(kf y), sin(kf x)) */
+    double dt;       /* resident stepping only */
+} tara_com_params;
+
+/* Replaces src/derive2com.f95:5 derive2com(Nx,Ny,Nh,pi,time,rho_k,rho_ux_k,
+ * rho_uy_k,E_k,Bx_k,By_k, 12 interleaved (d_*_dt_old, d_*_dt_new)):
+ * state = {rho, rho ux, rho uy, E, Bx, By} (in), rhs = the six d_*_dt_new (out).
+ * The forcing uses x(i) = (i-1) Lx/Nx, y(j) = (j-1) Ly/Ny, the grid of
+ * src/main2com.f95:231-233 (the routine's own x, y are uninitialised,
+ * src/derive2com.f95:15).  curl B is formed as the reference writes it
+ * (from the velocity derivatives divided by Nx Ny once more,
+ * src/derive2com.f95:172-175). */
+int tara_derive2com(int nx, int ny, int nh, const tara_com_params* p, const double* const state[6],
+                    double* const rhs[6]);
+/* Replaces src/ab2com.f95:6 (AB2 on all six fields). */
+int tara_ab2com(int nx, int ny, int nh, double dt, const double* const state[6], double* const state_new[6],
+                const double* const f_old[6], const double* const f_new[6]);
+
+/* Device-resident AB2 stepping of either family (the derive + ab2 + rotation
+ * loops of src/main2coh.f95:278-300 and src/main2com.f95:364-380, without the
+ * diagnostics).  upload/download take 3 (coh) or 6 (com) arrays; the AB2
+ * history starts at zero on upload.  Timing classes: 0 = prep + inverse y,
+ * 1 = x passes (+ the com middle pass), 2 = forward y + assembly. */
+typedef struct tara_f2 tara_f2;
+int tara_coh_create(int nx, int ny, const tara_coh_params* p, tara_f2** out);
+int tara_com_create(int nx, int ny, const tara_com_params* p, tara_f2** out);
+int tara_f2_upload(tara_f2* ctx, const double* const state[]);
+int tara_f2_download(tara_f2* ctx, double* const state[]);
+int tara_f2_advance(tara_f2* ctx, int nsteps);
+int tara_f2_sync(tara_f2* ctx);
+int tara_f2_destroy(tara_f2* ctx);
+int tara_f2_set_timing(tara_f2* ctx, int on);
+int tara_f2_kernel_stats(tara_f2* ctx, int nclass, long* counts, double* ms);
+
 /* ------------------------------------ 3D compressible isothermal MHD (3com)
  * Fields, in the order of the reference's argument lists: rho, rho_ux, rho_uy,
  * rho_uz, E, Bx, By, Bz; each a (Nh, Ny, Nz) half spectrum.  Physical box

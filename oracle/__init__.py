@@ -50,7 +50,11 @@ def lib():
         _orc.orc_ab2csh.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         _orc.orc_ab2csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
         _orc.orc_rk4csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+        _orc.orc_derive2css.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
+        _orc.orc_rk4css_steps.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [C.c_int, _dp]
         _orc.orc_ic2csh.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5 + [_dp, _dp]
+        _orc.orc_derive2coh.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _orc.orc_derive2com.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _orc.orc_derive3com.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _orc.orc_ab3com.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 4
     return _orc
@@ -140,6 +144,19 @@ def rk4csh_steps(w, nx, nu, dt, nsteps):
     return w
 
 
+def derive2css(w, nx, alpha, nu, lam):
+    w = np.asfortranarray(w, dtype=np.complex128)
+    out = np.zeros_like(w, order="F")
+    lib().orc_derive2css(nx, w.shape[1], alpha, nu, lam, _ptr(w), _ptr(out))
+    return out
+
+
+def rk4css_steps(w, nx, alpha, nu, lam, dt, nsteps):
+    w = np.array(w, dtype=np.complex128, order="F")
+    lib().orc_rk4css_steps(nx, w.shape[1], alpha, nu, lam, dt, nsteps, _ptr(w))
+    return w
+
+
 def ic2csh(nx, ny, lx_in=2.0, ly_in=2.0, w0=25.0, m0=2.0, d_in=3.0):
     omega = np.zeros((nx, ny), order="F")
     wk = zeros_spec(nx, ny)
@@ -220,6 +237,124 @@ def ref_rk4csh_steps(w, nx, nu, dt, nsteps):
     w = np.array(w, dtype=np.complex128, order="F")
     ref().ref_rk4csh_steps(nx, w.shape[1], nu, dt, nsteps, _ptr(w))
     return w
+
+
+class CohParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("pi", "lx", "ly", "mux", "muy", "cs")]
+
+
+class ComParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("pi", "lx", "ly", "mu", "eta", "cs", "kf", "a")]
+
+
+def derive2coh(state, nx, pi, lx, ly, mux, muy, cs):
+    state = [np.array(s, dtype=np.complex128, order="F") for s in state]
+    out = [np.zeros_like(state[0], order="F") for _ in range(3)]
+    prm = CohParams(pi, lx, ly, mux, muy, cs)
+    lib().orc_derive2coh(nx, state[0].shape[1], C.byref(prm), _ptr_array(state), _ptr_array(out))
+    return out
+
+
+def derive2com(state, nx, pi, lx, ly, mu, eta, cs, kf, a):
+    state = [np.array(s, dtype=np.complex128, order="F") for s in state]
+    out = [np.zeros_like(state[0], order="F") for _ in range(6)]
+    prm = ComParams(pi, lx, ly, mu, eta, cs, kf, a)
+    lib().orc_derive2com(nx, state[0].shape[1], C.byref(prm), _ptr_array(state), _ptr_array(out))
+    return out
+
+
+class _CommCoh(C.Structure):
+    """COMMON /comm/ as src/main2coh.f95:29 lays it out."""
+    _fields_ = [(n, C.c_double) for n in ("Lx", "Ly", "mux", "muy", "ms", "CS", "dt")]
+
+
+class _CommCom(C.Structure):
+    """COMMON /comm/ as src/main2com.f95:43 lays it out."""
+    _fields_ = [(n, C.c_double) for n in ("Lx", "Ly", "spheat", "mu", "ms", "CS", "mu_0", "eta", "dt", "kf", "A")]
+
+
+def ref_derive2coh(state, nx, pi, lx, ly, mux, muy, cs):
+    """Calls the reference derive2coh (COMMON written with the coh layout)."""
+    state = [np.array(s, dtype=np.complex128, order="F") for s in state]
+    nh, ny = state[0].shape
+    cm = _CommCoh.in_dll(ref(), "comm_")
+    cm.Lx, cm.Ly, cm.mux, cm.muy, cm.ms, cm.CS, cm.dt = lx, ly, mux, muy, 1.0, cs, 0.0
+    old = [np.zeros_like(state[0], order="F") for _ in range(3)]
+    new = [np.zeros_like(state[0], order="F") for _ in range(3)]
+    ref().derive2coh_(_i(nx), _i(ny), _i(nh), _d(pi), _d(0.0), *[_ptr(s) for s in state],
+                      *[_ptr(o) for o in old], *[_ptr(n) for n in new])
+    return new
+
+
+def ref_derive2com(state, nx, pi, lx, ly, mu, eta, cs, kf=0.0, a=0.0):
+    """Calls the reference derive2com (COMMON written with the com layout).  Its
+    forcing reads uninitialised x, y arrays; with A = kf = 0 the forcing terms
+    are zero whatever those hold (unless they hold non-finite values, which the
+    caller checks)."""
+    state = [np.array(s, dtype=np.complex128, order="F") for s in state]
+    nh, ny = state[0].shape
+    cm = _CommCom.in_dll(ref(), "comm_")
+    cm.Lx, cm.Ly, cm.spheat, cm.mu, cm.ms, cm.CS = lx, ly, 5.0 / 3.0, mu, 1.0, cs
+    cm.mu_0, cm.eta, cm.dt, cm.kf, cm.A = 1.0, eta, 0.0, kf, a
+    old = [np.zeros_like(state[0], order="F") for _ in range(6)]
+    new = [np.zeros_like(state[0], order="F") for _ in range(6)]
+    args = [_i(nx), _i(ny), _i(nh), _d(pi), _d(0.0)] + [_ptr(s) for s in state]
+    for o, n in zip(old, new):
+        args += [_ptr(o), _ptr(n)]
+    ref().derive2com_(*args)
+    return new
+
+
+def _ref_pointwise(name, nx, args_before, arrays, *, dt, comm):
+    """Calls a pointwise reference stepper after writing dt into COMMON."""
+    comm.dt = dt
+    getattr(ref(), name)(*args_before, *[_ptr(a) for a in arrays])
+
+
+def ref_ab2coh(state, f_old, f_new, nx, dt):
+    state, f_old, f_new = ([np.array(s, dtype=np.complex128, order="F") for s in x] for x in (state, f_old, f_new))
+    nh, ny = state[0].shape
+    new = [np.zeros_like(state[0], order="F") for _ in range(3)]
+    cm = _CommCoh.in_dll(ref(), "comm_")
+    _ref_pointwise("ab2coh_", nx, [_i(nx), _i(ny), _i(nh), _d(np.pi), _d(0.0)],
+                   state + new + f_old + f_new, dt=dt, comm=cm)
+    return new
+
+
+def ref_euler2coh(state, slope, nx, dt):
+    state, slope = ([np.array(s, dtype=np.complex128, order="F") for s in x] for x in (state, slope))
+    nh, ny = state[0].shape
+    new = [np.zeros_like(state[0], order="F") for _ in range(3)]
+    cm = _CommCoh.in_dll(ref(), "comm_")
+    _ref_pointwise("euler2coh_", nx, [_i(nx), _i(ny), _i(nh), _d(np.pi), _d(0.0)],
+                   state + new + slope, dt=dt, comm=cm)
+    return new
+
+
+def ref_nk_stepper(name, state, slope, nx, dt, pi, lx, ly, mux, muy, cs):
+    """rk4_nk2coh_ / pc_nk2coh_: new density from (nk, ukx, uky) and the slopes.
+    Their internal derive2coh calls run on the given state (results unused)."""
+    state, slope = ([np.array(s, dtype=np.complex128, order="F") for s in x] for x in (state, slope))
+    nh, ny = state[0].shape
+    cm = _CommCoh.in_dll(ref(), "comm_")
+    cm.Lx, cm.Ly, cm.mux, cm.muy, cm.ms, cm.CS, cm.dt = lx, ly, mux, muy, 1.0, cs, dt
+    nk_new = np.zeros_like(state[0], order="F")
+    getattr(ref(), name)(_i(nx), _i(ny), _i(nh), _d(pi), _d(0.0), _ptr(state[0]), _ptr(nk_new), _ptr(state[1]),
+                         _ptr(state[2]), *[_ptr(s) for s in slope])
+    return nk_new
+
+
+def ref_ab2com(state, f_old, f_new, nx, dt):
+    state, f_old, f_new = ([np.array(s, dtype=np.complex128, order="F") for s in x] for x in (state, f_old, f_new))
+    nh, ny = state[0].shape
+    new = [np.zeros_like(state[0], order="F") for _ in range(6)]
+    cm = _CommCom.in_dll(ref(), "comm_")
+    cm.dt = dt
+    args = [_i(nx), _i(ny), _i(nh), _d(np.pi), _d(0.0)] + [_ptr(s) for s in state] + [_ptr(s) for s in new]
+    for o, n in zip(f_old, f_new):
+        args += [_ptr(o), _ptr(n)]
+    ref().ab2com_(*args)
+    return new
 
 
 class _Comm(C.Structure):

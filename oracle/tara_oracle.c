@@ -284,6 +284,299 @@ void orc_rk4csh_steps(int nx, int ny, double nu, double dt, int nsteps, double *
     free(k1); free(k2); free(k3); free(k4); free(s);
 }
 
+/* src/derive2css.f95:6-136, screened vorticity.  The reference leaves Lx, Ly
+   and lambda uninitialised (:11); this restatement takes Lx = Ly = 2 pi with
+   the routine's own pi = 3.1428 (:17), as derive2csh does, so kx, ky are the
+   integers of csh_k, and lambda as an argument.  psi(0,0) = 0 (:24-27);
+   omega_d = k^2 psi (:36, 48) carries the derivatives (:37-38) and the viscous
+   term (:124); the dealiased sum of the two products is scaled by alpha (:123). */
+void orc_derive2css(int nx, int ny, double alpha, double nu, double lambda, const double *w, double *dt_new)
+{
+    int nh = nx / 2 + 1;
+    size_t ns = (size_t)nh * ny, nr = (size_t)nx * ny;
+    double *ukx = calloc(2 * ns, 8), *uky = calloc(2 * ns, 8), *ikxw = calloc(2 * ns, 8), *ikyw = calloc(2 * ns, 8);
+    double *wd = calloc(2 * ns, 8);
+    double *ux = malloc(8 * nr), *uy = malloc(8 * nr), *wx = malloc(8 * nr), *wy = malloc(8 * nr);
+    double *nlkx = malloc(16 * ns), *nlky = malloc(16 * ns);
+    for (int j = 0; j < ny; ++j) {
+        double ky = csh_k(j, ny, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = csh_k(i, nx, 1);
+            size_t p = (size_t)j * nh + i;
+            double k2 = kx * kx + ky * ky;
+            double pr = 0.0, pim = 0.0;
+            if (!(i == 0 && j == 0)) { double den = k2 + lambda; pr = RE(w, p) / den; pim = IM(w, p) / den; }
+            RE(ukx, p) = -(ky * pim); IM(ukx, p) = ky * pr;       /* +i ky psi */
+            RE(uky, p) = kx * pim;    IM(uky, p) = -(kx * pr);    /* -i kx psi */
+            double dr = k2 * pr, di = k2 * pim;                   /* omega_d */
+            RE(wd, p) = dr; IM(wd, p) = di;
+            RE(ikxw, p) = -(kx * di); IM(ikxw, p) = kx * dr;
+            RE(ikyw, p) = -(ky * di); IM(ikyw, p) = ky * dr;
+        }
+    }
+    orc_c2r_2d(nx, ny, ukx, ux);
+    orc_c2r_2d(nx, ny, uky, uy);
+    orc_c2r_2d(nx, ny, ikxw, wx);
+    orc_c2r_2d(nx, ny, ikyw, wy);
+    double norm = (double)nx * (double)ny;
+    for (size_t q = 0; q < nr; ++q) {
+        double a = ux[q] / norm, b = uy[q] / norm, c = wx[q] / norm, d = wy[q] / norm;
+        wx[q] = a * c;
+        wy[q] = b * d;
+    }
+    orc_r2c_2d(nx, ny, wx, nlkx);
+    orc_r2c_2d(nx, ny, wy, nlky);
+    double cut = ((double)(nx + ny) / 2.0) / 3.0;
+    for (int j = 0; j < ny; ++j) {
+        double ky = csh_k(j, ny, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = csh_k(i, nx, 1);
+            size_t p = (size_t)j * nh + i;
+            double ar = RE(nlkx, p), ai = IM(nlkx, p), br = RE(nlky, p), bi = IM(nlky, p);
+            if (sqrt(kx * kx + ky * ky) >= cut) { ar = ai = br = bi = 0.0; }
+            double vr = nu * (kx * kx + ky * ky);
+            RE(dt_new, p) = -(alpha * (ar + br) + vr * RE(wd, p));
+            IM(dt_new, p) = -(alpha * (ai + bi) + vr * IM(wd, p));
+        }
+    }
+    free(ukx); free(uky); free(ikxw); free(ikyw); free(wd); free(ux); free(uy); free(wx); free(wy);
+    free(nlkx); free(nlky);
+}
+
+/* Classic RK4 over derive2css (the scheme src/rk42css.f95:15-48 intends). */
+void orc_rk4css_steps(int nx, int ny, double alpha, double nu, double lambda, double dt, int nsteps, double *w)
+{
+    size_t n2 = 2 * (size_t)(nx / 2 + 1) * ny;
+    double *k1 = malloc(8 * n2), *k2 = malloc(8 * n2), *k3 = malloc(8 * n2), *k4 = malloc(8 * n2), *s = malloc(8 * n2);
+    for (int t = 0; t < nsteps; ++t) {
+        orc_derive2css(nx, ny, alpha, nu, lambda, w, k1);
+        for (size_t q = 0; q < n2; ++q) s[q] = w[q] + (0.5 * dt) * k1[q];
+        orc_derive2css(nx, ny, alpha, nu, lambda, s, k2);
+        for (size_t q = 0; q < n2; ++q) s[q] = w[q] + (0.5 * dt) * k2[q];
+        orc_derive2css(nx, ny, alpha, nu, lambda, s, k3);
+        for (size_t q = 0; q < n2; ++q) s[q] = w[q] + dt * k3[q];
+        orc_derive2css(nx, ny, alpha, nu, lambda, s, k4);
+        for (size_t q = 0; q < n2; ++q)
+            w[q] = w[q] + (dt / 6.0) * (((k1[q] + 2.0 * k2[q]) + 2.0 * k3[q]) + k4[q]);
+    }
+    free(k1); free(k2); free(k3); free(k4); free(s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2D compressible families: src/derive2coh.f95, src/derive2com.f95          */
+/* ------------------------------------------------------------------------ */
+
+/* kx = 2 pi (i-1)/Lx, ky = 2 pi (j-1)/Ly or 2 pi ((j-1)-Ny)/Ly with the
+   caller's pi (src/derive2coh.f95:59-60, 74-75). */
+static double f2_k(int i, int n, double pi, double l, int half)
+{
+    int m = (half || i < n / 2) ? i : i - n;
+    return 2.0 * pi * (double)m / l;
+}
+
+static int f2_cut(double kx, double ky, int nx, int ny)
+{
+    return sqrt(kx * kx + ky * ky) >= ((double)(nx + ny) / 2.0) / 3.0 + 0;
+}
+
+/* spectral multiply helpers on interleaved complex arrays */
+static void spec_ik(const double *z, double *o, size_t p, double k)
+{
+    double re = z[2 * p], im = z[2 * p + 1];
+    o[2 * p] = -(k * im);
+    o[2 * p + 1] = k * re;
+}
+
+static void spec_k2(const double *z, double *o, size_t p, double k)
+{
+    double kk = k * k;
+    o[2 * p] = kk * z[2 * p];
+    o[2 * p + 1] = kk * z[2 * p + 1];
+}
+
+/* src/derive2coh.f95:5-265.  state = {nk, ukx, uky}, rhs = {dn, dux, duy}. */
+void orc_derive2coh(int nx, int ny, const orc_coh_params *prm, const double *const state[3],
+                    double *const rhs[3])
+{
+    int nh = nx / 2 + 1;
+    size_t ns = (size_t)nh * ny, nr = (size_t)nx * ny;
+    /* 13 spectra to invert (:56-129) */
+    double *sp[13], *re[13];
+    for (int f = 0; f < 13; ++f) { sp[f] = calloc(2 * ns, 8); re[f] = malloc(8 * nr); }
+    const double *nk = state[0], *ukx = state[1], *uky = state[2];
+    for (int j = 0; j < ny; ++j) {
+        double ky = f2_k(j, ny, prm->pi, prm->ly, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = f2_k(i, nx, prm->pi, prm->lx, 1);
+            size_t p = (size_t)j * nh + i;
+            sp[0][2 * p] = nk[2 * p]; sp[0][2 * p + 1] = nk[2 * p + 1];
+            sp[1][2 * p] = ukx[2 * p]; sp[1][2 * p + 1] = ukx[2 * p + 1];
+            sp[2][2 * p] = uky[2 * p]; sp[2][2 * p + 1] = uky[2 * p + 1];
+            spec_ik(nk, sp[3], p, kx);   /* i kx n  */
+            spec_ik(nk, sp[4], p, ky);   /* i ky n  */
+            spec_ik(ukx, sp[5], p, kx);  /* i kx ux */
+            spec_ik(ukx, sp[6], p, ky);  /* i ky ux */
+            spec_ik(uky, sp[7], p, kx);  /* i kx uy */
+            spec_ik(uky, sp[8], p, ky);  /* i ky uy */
+            spec_k2(ukx, sp[9], p, kx);  /* kx^2 ux */
+            spec_k2(ukx, sp[10], p, ky); /* ky^2 ux */
+            spec_k2(uky, sp[11], p, kx); /* kx^2 uy */
+            spec_k2(uky, sp[12], p, ky); /* ky^2 uy */
+        }
+    }
+    for (int f = 0; f < 13; ++f) orc_c2r_2d(nx, ny, sp[f], re[f]);
+    double norm = (double)nx * (double)ny;
+    /* 9 real-space terms (:131-167) */
+    double *t[9];
+    for (int f = 0; f < 9; ++f) t[f] = malloc(8 * nr);
+    for (size_t q = 0; q < nr; ++q) {
+        double n = re[0][q] / norm, ux = re[1][q] / norm, uy = re[2][q] / norm;
+        double dnx = re[3][q] / norm, dny = re[4][q] / norm;
+        double duxx = re[5][q] / norm, duxy = re[6][q] / norm, duyx = re[7][q] / norm, duyy = re[8][q] / norm;
+        double d2uxx = re[9][q] / norm, d2uxy = re[10][q] / norm, d2uyx = re[11][q] / norm, d2uyy = re[12][q] / norm;
+        t[0][q] = dnx / n;                  /* Px */
+        t[1][q] = dny / n;                  /* Py */
+        t[2][q] = d2uxx / n;                /* Viscxx */
+        t[3][q] = d2uxy / n;                /* Viscxy */
+        t[4][q] = d2uyx / n;                /* Viscyx */
+        t[5][q] = d2uyy / n;                /* Viscyy */
+        t[6][q] = ux * dnx + uy * dny + n * (duxx + duyy);  /* NLn */
+        t[7][q] = ux * duxx + uy * duxy;    /* NLx */
+        t[8][q] = ux * duyx + uy * duyy;    /* NLy */
+    }
+    double *tk[9];
+    for (int f = 0; f < 9; ++f) { tk[f] = malloc(16 * ns); orc_r2c_2d(nx, ny, t[f], tk[f]); }
+    /* dealias all (:201-242), assemble (:246-258) */
+    double cs2 = prm->cs * prm->cs;
+    for (int j = 0; j < ny; ++j) {
+        double ky = f2_k(j, ny, prm->pi, prm->ly, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = f2_k(i, nx, prm->pi, prm->lx, 1);
+            size_t p = (size_t)j * nh + i;
+            int cut = f2_cut(kx, ky, nx, ny);
+            for (int c = 0; c < 2; ++c) {
+                double v[9];
+                for (int f = 0; f < 9; ++f) v[f] = cut ? 0.0 : tk[f][2 * p + c];
+                double nlx = v[7] + (prm->mux * v[2] + prm->muy * v[3]) + cs2 * v[0];
+                double nly = v[8] + (prm->mux * v[4] + prm->muy * v[5]) + cs2 * v[1];
+                rhs[0][2 * p + c] = -v[6];
+                rhs[1][2 * p + c] = -nlx;
+                rhs[2][2 * p + c] = -nly;
+            }
+        }
+    }
+    for (int f = 0; f < 13; ++f) { free(sp[f]); free(re[f]); }
+    for (int f = 0; f < 9; ++f) { free(t[f]); free(tk[f]); }
+}
+
+/* src/derive2com.f95:5-426.  state = {rho, rho ux, rho uy, E, Bx, By}.  The
+   forcing uses x(i) = (i-1) Lx/Nx, y(j) = (j-1) Ly/Ny (src/main2com.f95:231-233;
+   the routine's own x, y are uninitialised).  curl B follows the reference's
+   lines :172-175 as written. */
+void orc_derive2com(int nx, int ny, const orc_com_params *prm, const double *const state[6],
+                    double *const rhs[6])
+{
+    int nh = nx / 2 + 1;
+    size_t ns = (size_t)nh * ny, nr = (size_t)nx * ny;
+    double norm = (double)nx * (double)ny;
+    double *r[6];
+    for (int f = 0; f < 6; ++f) { r[f] = malloc(8 * nr); orc_c2r_2d(nx, ny, state[f], r[f]); }
+    double *ux = malloc(8 * nr), *uy = malloc(8 * nr), *fx = malloc(8 * nr), *fy = malloc(8 * nr);
+    double dx = prm->lx / (double)nx, dy = prm->ly / (double)ny;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t q = (size_t)j * nx + i;
+            for (int f = 0; f < 6; ++f) r[f][q] = r[f][q] / norm;
+            ux[q] = r[1][q] / r[0][q];
+            uy[q] = r[2][q] / r[0][q];
+            fx[q] = -(r[0][q] * prm->a * sin(prm->kf * ((double)j * dy)));
+            fy[q] = r[0][q] * prm->a * sin(prm->kf * ((double)i * dx));
+        }
+    double *uxk = malloc(16 * ns), *uyk = malloc(16 * ns), *fxk = malloc(16 * ns), *fyk = malloc(16 * ns);
+    orc_r2c_2d(nx, ny, ux, uxk);
+    orc_r2c_2d(nx, ny, uy, uyk);
+    orc_r2c_2d(nx, ny, fx, fxk);
+    orc_r2c_2d(nx, ny, fy, fyk);
+    double *a = calloc(2 * ns, 8), *b = calloc(2 * ns, 8);
+    for (int j = 0; j < ny; ++j) {
+        double ky = f2_k(j, ny, prm->pi, prm->ly, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = f2_k(i, nx, prm->pi, prm->lx, 1);
+            size_t p = (size_t)j * nh + i;
+            spec_ik(uxk, a, p, kx);
+            spec_ik(uyk, b, p, ky);
+        }
+    }
+    double *duxdx = malloc(8 * nr), *duydy = malloc(8 * nr);
+    orc_c2r_2d(nx, ny, a, duxdx);
+    orc_c2r_2d(nx, ny, b, duydy);
+    /* fluxes (:169-226) */
+    double *fl[9];
+    for (int f = 0; f < 9; ++f) fl[f] = malloc(8 * nr);
+    double cs2 = prm->cs * prm->cs;
+    for (size_t q = 0; q < nr; ++q) {
+        double rho = r[0][q], rux = r[1][q], ruy = r[2][q], E = r[3][q], bx = r[4][q], by = r[5][q];
+        double vx = ux[q], vy = uy[q];
+        double b2 = bx * bx + by * by;
+        double dudx = duxdx[q] / norm, dvdy = duydy[q] / norm;
+        double dbxdy = dudx / norm, dbydx = dvdy / norm;
+        double curl = dbydx - dbxdy;
+        double P = cs2 * rho;
+        fl[0][q] = rux * vx + P + b2 / 2.0 - bx * bx;   /* Mom_x_1 */
+        fl[1][q] = rux * vy - bx * by;                  /* Mom_x_2 */
+        fl[2][q] = rux * vy - bx * by;                  /* Mom_y_1 */
+        fl[3][q] = ruy * vy + P + b2 / 2.0 - by * by;   /* Mom_y_2 */
+        double ex = (E + P + b2 / 2.0) * vx - vx * bx * bx - vy * bx * by;
+        fl[4][q] = ex - prm->eta * by * curl;           /* Energy_x */
+        double ey = (E + P + b2 / 2.0) * vy - vx * bx * by - vy * by * by;
+        fl[5][q] = ey + prm->eta * bx * curl;           /* Energy_y */
+        double dv = dudx + dvdy;
+        fl[6][q] = dv * dv;                             /* E_Visc */
+        fl[7][q] = vy * bx - by * vx;                   /* Mag_x */
+        fl[8][q] = vx * by - bx * vy;                   /* Mag_y */
+    }
+    double *fk[9];
+    for (int f = 0; f < 9; ++f) { fk[f] = malloc(16 * ns); orc_r2c_2d(nx, ny, fl[f], fk[f]); }
+    /* spectral assembly (:252-420) */
+    for (int j = 0; j < ny; ++j) {
+        double ky = f2_k(j, ny, prm->pi, prm->ly, 0);
+        for (int i = 0; i < nh; ++i) {
+            double kx = f2_k(i, nx, prm->pi, prm->lx, 1);
+            size_t p = (size_t)j * nh + i;
+            int cut = f2_cut(kx, ky, nx, ny);
+            double t[19][2];
+#define IK(src, k, slot) do { double re_ = (src)[2 * p], im_ = (src)[2 * p + 1]; \
+            t[slot][0] = -((k) * im_); t[slot][1] = (k) * re_; } while (0)
+#define K2(src, k, slot) do { double kk_ = (k) * (k); t[slot][0] = kk_ * (src)[2 * p]; t[slot][1] = kk_ * (src)[2 * p + 1]; } while (0)
+            IK(state[1], kx, 0);  IK(state[2], ky, 1);
+            IK(fk[0], kx, 2);     IK(fk[1], ky, 3);     IK(fk[2], kx, 4);   IK(fk[3], ky, 5);
+            K2(uxk, kx, 6);       K2(uxk, ky, 7);       K2(uyk, kx, 8);     K2(uyk, ky, 9);
+            IK(fk[4], kx, 10);    IK(fk[5], ky, 11);
+            t[12][0] = fk[6][2 * p]; t[12][1] = fk[6][2 * p + 1];
+            IK(fk[7], ky, 13);    IK(fk[8], kx, 14);
+            K2(state[4], kx, 15); K2(state[4], ky, 16); K2(state[5], kx, 17); K2(state[5], ky, 18);
+#undef IK
+#undef K2
+            if (cut) for (int s_ = 0; s_ < 19; ++s_) t[s_][0] = t[s_][1] = 0.0;
+            for (int c = 0; c < 2; ++c) {
+                rhs[0][2 * p + c] = -(t[0][c] + t[1][c]);
+                double d1 = -(t[2][c] + t[3][c]);
+                rhs[1][2 * p + c] = d1 - prm->mu * (t[6][c] + t[7][c]) + fxk[2 * p + c];
+                double d2 = -(t[4][c] + t[5][c]);
+                rhs[2][2 * p + c] = d2 - prm->mu * (t[8][c] + t[9][c]) + fyk[2 * p + c];
+                double d3 = -(t[10][c] + t[11][c]);
+                rhs[3][2 * p + c] = d3 + prm->mu * t[12][c];
+                rhs[4][2 * p + c] = -t[13][c] - prm->eta * (t[15][c] + t[16][c]);
+                rhs[5][2 * p + c] = -t[14][c] - prm->eta * (t[17][c] + t[18][c]);
+            }
+        }
+    }
+    for (int f = 0; f < 6; ++f) free(r[f]);
+    for (int f = 0; f < 9; ++f) { free(fl[f]); free(fk[f]); }
+    free(ux); free(uy); free(fx); free(fy); free(uxk); free(uyk); free(fxk); free(fyk);
+    free(a); free(b); free(duxdx); free(duydy);
+}
+
 /* Shear-layer IC of src/main2csh.f95:84-110 (x-fastest real field, then r2c). */
 void orc_ic2csh(int nx, int ny, double lx_in, double ly_in, double w0, double m0,
                 double d_in, double *omega, double *omegak)

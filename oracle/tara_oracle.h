@@ -29,8 +29,26 @@ void orc_ab2csh(int nx, int ny, double dt, const double *omegak, double *omegak_
 void orc_ab2csh_steps(int nx, int ny, double nu, double dt, int nsteps,
                       double *omegak, double *hist);
 void orc_rk4csh_steps(int nx, int ny, double nu, double dt, int nsteps, double *omegak);
+/* ---- 2D screened vorticity (derive2css family; Lx = Ly = 2 pi, lambda given) ---- */
+void orc_derive2css(int nx, int ny, double alpha, double nu, double lambda, const double *omegak, double *dt_new);
+void orc_rk4css_steps(int nx, int ny, double alpha, double nu, double lambda, double dt, int nsteps,
+                      double *omegak);
 void orc_ic2csh(int nx, int ny, double lx_in, double ly_in, double w0, double m0,
                 double d_in, double *omega, double *omegak);
+
+/* ---- 2D compressible families (derive2coh, derive2com) ---- */
+typedef struct {
+    double pi, lx, ly;   /* pi argument; box already multiplied by pi */
+    double mux, muy, cs;
+} orc_coh_params;
+typedef struct {
+    double pi, lx, ly;
+    double mu, eta, cs, kf, a;
+} orc_com_params;
+void orc_derive2coh(int nx, int ny, const orc_coh_params *p, const double *const state[3],
+                    double *const rhs[3]);
+void orc_derive2com(int nx, int ny, const orc_com_params *p, const double *const state[6],
+                    double *const rhs[6]);
 
 /* ---- 3D compressible isothermal MHD (derive3com / ab3com) ---- */
 typedef struct {

@@ -6,6 +6,12 @@ array layout follow the Fortran routines they replace:
 
   derive2csh(Nx, Ny, Nh, nu, omegak)            src/derive2csh.f95:3
   ab2csh(Nx, Ny, Nh, dt, omegak, f_old, f_new)  src/ab2csh.f95:3
+  derive2css(Nx, Ny, Nh, alpha, nu, omegak, lam)   src/derive2css.f95:6
+  ab2css / rk42css                              src/ab2css.f95:6, src/rk42css.f95:6
+  derive2coh / ab2coh / euler2coh / rk4coh_field / pccoh_field
+                                                src/derive2coh.f95:5, ab2coh.f95:5, euler2coh.f95:5,
+                                                rk4nk2coh.f95:6, pcnk2coh.f95:6
+  derive2com / ab2com                           src/derive2com.f95:5, src/ab2com.f95:6
   derive3com(Nx, Ny, Nz, Nh, params, state)     src/derive3com.f95:5
   ab3com(Nx, Ny, Nz, Nh, dt, state, f_old, f_new)  src/ab3com.f95:6
   fft_r2c / fft_c2r                             dfftw_plan_dft_{r2c,c2r}_{2d,3d} + execute
@@ -38,6 +44,16 @@ class MhdParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("lx", "ly", "lz", "mu", "eta", "cs", "dt")]
 
 
+class CohParams(C.Structure):
+    """tara_coh_params: the pi argument, COMMON Lx, Ly (times pi), mux, muy, CS, dt."""
+    _fields_ = [(n, C.c_double) for n in ("pi", "lx", "ly", "mux", "muy", "cs", "dt")]
+
+
+class ComParams(C.Structure):
+    """tara_com_params: the pi argument, COMMON Lx, Ly, mu, eta, CS, kf, A, dt."""
+    _fields_ = [(n, C.c_double) for n in ("pi", "lx", "ly", "mu", "eta", "cs", "kf", "a", "dt")]
+
+
 def lib():
     """Loads libtara_hip.so (built by ``python -m tara_amd.build``)."""
     global _lib
@@ -53,6 +69,10 @@ def lib():
         L.tara_derive2csh.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp]
         L.tara_ab2csh.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         L.tara_csh_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
+        L.tara_derive2css.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.tara_ab2css.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        L.tara_rk42css.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 4 + [_dp, _dp]
+        L.tara_css_create.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [C.c_int, C.POINTER(C.c_void_p)]
         L.tara_csh_upload.argtypes = [C.c_void_p, _dp]
         L.tara_csh_download.argtypes = [C.c_void_p, _dp]
         L.tara_csh_advance.argtypes = [C.c_void_p, C.c_int]
@@ -60,6 +80,23 @@ def lib():
         L.tara_csh_destroy.argtypes = [C.c_void_p]
         L.tara_csh_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.tara_csh_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
+        P3, P6 = C.c_void_p * 3, C.c_void_p * 6
+        L.tara_derive2coh.argtypes = [C.c_int] * 3 + [C.POINTER(CohParams), P3, P3]
+        L.tara_ab2coh.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, P3, P3]
+        L.tara_euler2coh.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, P3]
+        for n in ("tara_rk4coh_field", "tara_pccoh_field"):
+            getattr(L, n).argtypes = [C.c_int] * 3 + [C.c_double, _dp, _dp, _dp]
+        L.tara_derive2com.argtypes = [C.c_int] * 3 + [C.POINTER(ComParams), P6, P6]
+        L.tara_ab2com.argtypes = [C.c_int] * 3 + [C.c_double, P6, P6, P6, P6]
+        L.tara_coh_create.argtypes = [C.c_int, C.c_int, C.POINTER(CohParams), C.POINTER(C.c_void_p)]
+        L.tara_com_create.argtypes = [C.c_int, C.c_int, C.POINTER(ComParams), C.POINTER(C.c_void_p)]
+        L.tara_f2_upload.argtypes = [C.c_void_p, C.c_void_p]
+        L.tara_f2_download.argtypes = [C.c_void_p, C.c_void_p]
+        L.tara_f2_advance.argtypes = [C.c_void_p, C.c_int]
+        for n in ("tara_f2_sync", "tara_f2_destroy"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.tara_f2_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.tara_f2_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
         P8 = C.c_void_p * 8
         L.tara_derive3com.argtypes = [C.c_int] * 4 + [C.POINTER(MhdParams), P8, P8]
         L.tara_ab3com.argtypes = [C.c_int] * 4 + [C.c_double, P8, P8, P8, P8]
@@ -139,13 +176,47 @@ def ab2csh(Nx, Ny, Nh, dt, omegak, dt_omegak_old, dt_omegak_new):
     return out
 
 
-class CshSolver:
-    """Device-resident 2D vorticity solver (the main2csh.f95:114-126 loop on the GPU)."""
+# lambda of the screened form when the caller gives none: main2css.f95:87
+CSS_LAMBDA = 10.0
 
-    def __init__(self, nx, ny, nu, dt, scheme=SCHEME_AB2):
+
+def derive2css(Nx, Ny, Nh, alpha, nu, omegak, lam=CSS_LAMBDA):
+    """dt_omegak_new = derive2css(...) (src/derive2css.f95:6); Lx = Ly = 2 pi."""
+    w = _spec(omegak)
+    assert w.shape == (Nh, Ny)
+    out = np.zeros_like(w, order="F")
+    _check(lib().tara_derive2css(Nx, Ny, Nh, alpha, nu, lam, _p(w), _p(out)))
+    return out
+
+
+def ab2css(Nx, Ny, Nh, dt, omegak, dt_omegak_old, dt_omegak_new):
+    """omegak_new = ab2css(...) (src/ab2css.f95:6)."""
+    w, fo, fn = _spec(omegak), _spec(dt_omegak_old), _spec(dt_omegak_new)
+    out = np.zeros_like(w, order="F")
+    _check(lib().tara_ab2css(Nx, Ny, Nh, dt, _p(w), _p(out), _p(fo), _p(fn)))
+    return out
+
+
+def rk42css(Nx, Ny, Nh, alpha, nu, dt, omegak, lam=CSS_LAMBDA):
+    """omegak_new = one classic RK4 step of derive2css (the intent of src/rk42css.f95:6)."""
+    w = _spec(omegak)
+    out = np.zeros_like(w, order="F")
+    _check(lib().tara_rk42css(Nx, Ny, Nh, alpha, nu, lam, dt, _p(w), _p(out)))
+    return out
+
+
+class CshSolver:
+    """Device-resident 2D vorticity solver (the main2csh.f95:114-126 loop on the
+    GPU).  With ``screened=True`` it steps derive2css's equations instead
+    (alpha, lam as in derive2css)."""
+
+    def __init__(self, nx, ny, nu, dt, scheme=SCHEME_AB2, screened=False, alpha=1.0, lam=CSS_LAMBDA):
         self.nx, self.ny, self.nh = nx, ny, nx // 2 + 1
         h = C.c_void_p()
-        _check(lib().tara_csh_create(nx, ny, nu, dt, scheme, C.byref(h)))
+        if screened:
+            _check(lib().tara_css_create(nx, ny, alpha, nu, lam, dt, scheme, C.byref(h)))
+        else:
+            _check(lib().tara_csh_create(nx, ny, nu, dt, scheme, C.byref(h)))
         self._h = h
 
     def upload(self, omegak):
@@ -176,6 +247,122 @@ class CshSolver:
     def close(self):
         if self._h:
             lib().tara_csh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------- 2D compressible families (coh, com)
+def _pn(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def _specs(arrs, shape):
+    out = [_spec(a) for a in arrs]
+    assert all(a.shape == shape for a in out)
+    return out
+
+
+def derive2coh(Nx, Ny, Nh, params, state):
+    """[dn_dt_new, dux_dt_new, duy_dt_new] = derive2coh(...) (src/derive2coh.f95:5)."""
+    st = _specs(state, (Nh, Ny))
+    out = [np.zeros((Nh, Ny), np.complex128, order="F") for _ in range(3)]
+    _check(lib().tara_derive2coh(Nx, Ny, Nh, C.byref(params), _pn(st), _pn(out)))
+    return out
+
+
+def ab2coh(Nx, Ny, Nh, dt, state, f_old, f_new):
+    st, fo, fn = (_specs(x, (Nh, Ny)) for x in (state, f_old, f_new))
+    out = [np.zeros((Nh, Ny), np.complex128, order="F") for _ in range(3)]
+    _check(lib().tara_ab2coh(Nx, Ny, Nh, dt, _pn(st), _pn(out), _pn(fo), _pn(fn)))
+    return out
+
+
+def euler2coh(Nx, Ny, Nh, dt, state, slope):
+    st, sl = (_specs(x, (Nh, Ny)) for x in (state, slope))
+    out = [np.zeros((Nh, Ny), np.complex128, order="F") for _ in range(3)]
+    _check(lib().tara_euler2coh(Nx, Ny, Nh, dt, _pn(st), _pn(out), _pn(sl)))
+    return out
+
+
+def rk4coh_field(Nx, Ny, Nh, dt, field, slope):
+    """The field update of rk4_nk2coh / rk4_ukx2coh / rk4_uky2coh."""
+    w, s_ = _spec(field), _spec(slope)
+    out = np.zeros_like(w, order="F")
+    _check(lib().tara_rk4coh_field(Nx, Ny, Nh, dt, _p(w), _p(out), _p(s_)))
+    return out
+
+
+def pccoh_field(Nx, Ny, Nh, dt, field, slope):
+    """The field update of pc_nk2coh / pc_ukx2coh / pc_uky2coh."""
+    w, s_ = _spec(field), _spec(slope)
+    out = np.zeros_like(w, order="F")
+    _check(lib().tara_pccoh_field(Nx, Ny, Nh, dt, _p(w), _p(out), _p(s_)))
+    return out
+
+
+def derive2com(Nx, Ny, Nh, params, state):
+    """The six d_*_dt_new of derive2com(...) (src/derive2com.f95:5)."""
+    st = _specs(state, (Nh, Ny))
+    out = [np.zeros((Nh, Ny), np.complex128, order="F") for _ in range(6)]
+    _check(lib().tara_derive2com(Nx, Ny, Nh, C.byref(params), _pn(st), _pn(out)))
+    return out
+
+
+def ab2com(Nx, Ny, Nh, dt, state, f_old, f_new):
+    st, fo, fn = (_specs(x, (Nh, Ny)) for x in (state, f_old, f_new))
+    out = [np.zeros((Nh, Ny), np.complex128, order="F") for _ in range(6)]
+    _check(lib().tara_ab2com(Nx, Ny, Nh, dt, _pn(st), _pn(out), _pn(fo), _pn(fn)))
+    return out
+
+
+class F2Solver:
+    """Device-resident AB2 stepping of derive2coh (params: CohParams) or
+    derive2com (params: ComParams)."""
+
+    def __init__(self, nx, ny, params):
+        self.nx, self.ny, self.nh = nx, ny, nx // 2 + 1
+        h = C.c_void_p()
+        if isinstance(params, CohParams):
+            self.nf = 3
+            _check(lib().tara_coh_create(nx, ny, C.byref(params), C.byref(h)))
+        else:
+            self.nf = 6
+            _check(lib().tara_com_create(nx, ny, C.byref(params), C.byref(h)))
+        self._h = h
+
+    def upload(self, state):
+        st = _specs(state, (self.nh, self.ny))
+        assert len(st) == self.nf
+        _check(lib().tara_f2_upload(self._h, _pn(st)))
+
+    def download(self):
+        out = [np.zeros((self.nh, self.ny), np.complex128, order="F") for _ in range(self.nf)]
+        _check(lib().tara_f2_download(self._h, _pn(out)))
+        return out
+
+    def advance(self, nsteps):
+        _check(lib().tara_f2_advance(self._h, nsteps))
+
+    def sync(self):
+        _check(lib().tara_f2_sync(self._h))
+
+    def set_timing(self, on):
+        _check(lib().tara_f2_set_timing(self._h, 1 if on else 0))
+
+    def kernel_stats(self, nclass=3):
+        counts = (C.c_long * nclass)()
+        ms = (C.c_double * nclass)()
+        _check(lib().tara_f2_kernel_stats(self._h, nclass, counts, ms))
+        return list(counts), list(ms)
+
+    def close(self):
+        if self._h:
+            lib().tara_f2_destroy(self._h)
             self._h = None
 
     def __del__(self):

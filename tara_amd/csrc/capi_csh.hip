@@ -1,5 +1,6 @@
-// C ABI of the 2D incompressible (csh) path: drop-in derive2csh / ab2csh on
-// host arrays, and the device-resident stepping context.
+// C ABI of the 2D incompressible paths: drop-in derive2csh / ab2csh and
+// derive2css / ab2css / rk42css on host arrays, and the device-resident
+// stepping context shared by both.
 #include <cstring>
 #include <map>
 #include <memory>
@@ -64,6 +65,7 @@ namespace {
 struct DropIn {
     CshPlan plan;
     DevBuf w, f, stage;  // w, f ky-major; stage in the caller's kx-fastest layout
+    DevBuf aux, next;    // RK4 accumulator and stage field (allocated on first RK4 call)
 };
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, std::unique_ptr<DropIn>> g_dropin;
@@ -92,15 +94,10 @@ struct Ab2Bufs {
 };
 Ab2Bufs g_ab2;
 
-}  // namespace
-}  // namespace tara
-
-using namespace tara;
-
-extern "C" int tara_derive2csh(int nx, int ny, int nh, double nu, const double* omegak, double* dt_new)
+int derive_dropin(const char* name, int nx, int ny, int nh, const CshPhys& ph, const double* omegak, double* dt_new)
 {
-    TARA_CHECK(omegak && dt_new, TARA_ERR_ARG, "derive2csh: null array");
-    TARA_CHECK(nh == nx / 2 + 1, TARA_ERR_ARG, "derive2csh: Nh=%d but Nx/2+1=%d", nh, nx / 2 + 1);
+    TARA_CHECK(omegak && dt_new, TARA_ERR_ARG, "%s: null array", name);
+    TARA_CHECK(nh == nx / 2 + 1, TARA_ERR_ARG, "%s: Nh=%d but Nx/2+1=%d", name, nh, nx / 2 + 1);
     std::lock_guard<std::mutex> lk(g_mu);
     DropIn* d = nullptr;
     int rc = get_dropin(nx, ny, &d);
@@ -109,12 +106,78 @@ extern "C" int tara_derive2csh(int nx, int ny, int nh, double nu, const double* 
     const size_t bytes = (size_t)nh * ny * sizeof(double2);
     TARA_HIP(hipMemcpy(d->stage.p, omegak, bytes, hipMemcpyHostToDevice));
     if ((rc = to_kymajor(d->stage.as<double2>(), d->w.as<double2>(), nh, ny, 1, nullptr))) return rc;
-    rc = csh_rhs_stage(p, d->w.as<double2>(), CSH_RHS, nu, 0.0, nullptr, nullptr, d->f.as<double2>(), nullptr,
+    rc = csh_rhs_stage(p, d->w.as<double2>(), CSH_RHS, ph, 0.0, nullptr, nullptr, d->f.as<double2>(), nullptr,
                        nullptr);
     if (rc) return rc;
     if ((rc = from_kymajor(d->f.as<double2>(), d->stage.as<double2>(), nh, ny, 1, nullptr))) return rc;
     TARA_HIP(hipMemcpy(dt_new, d->stage.p, bytes, hipMemcpyDeviceToHost));
     return TARA_OK;
+}
+
+// One classic RK4 step of the RHS selected by ph on host arrays.
+int rk4_dropin(const char* name, int nx, int ny, int nh, const CshPhys& ph, double dt, const double* omegak,
+               double* omegak_new)
+{
+    TARA_CHECK(omegak && omegak_new, TARA_ERR_ARG, "%s: null array", name);
+    TARA_CHECK(nh == nx / 2 + 1, TARA_ERR_ARG, "%s: Nh=%d but Nx/2+1=%d", name, nh, nx / 2 + 1);
+    std::lock_guard<std::mutex> lk(g_mu);
+    DropIn* d = nullptr;
+    int rc = get_dropin(nx, ny, &d);
+    if (rc) return rc;
+    const CshPlan& p = d->plan;
+    const size_t bytes = (size_t)nh * ny * sizeof(double2);
+    if (!d->aux.p && ((rc = d->aux.alloc(bytes)) || (rc = d->next.alloc(bytes)))) return rc;
+    double2* w = d->w.as<double2>();
+    double2* aux = d->aux.as<double2>();
+    double2* nxt = d->next.as<double2>();
+    TARA_HIP(hipMemcpy(d->stage.p, omegak, bytes, hipMemcpyHostToDevice));
+    if ((rc = to_kymajor(d->stage.as<double2>(), w, nh, ny, 1, nullptr))) return rc;
+    if ((rc = csh_rhs_stage(p, w, CSH_RK4_1, ph, dt, w, aux, nxt, nullptr, nullptr))) return rc;
+    if ((rc = csh_rhs_stage(p, nxt, CSH_RK4_2, ph, dt, w, aux, nxt, nullptr, nullptr))) return rc;
+    if ((rc = csh_rhs_stage(p, nxt, CSH_RK4_3, ph, dt, w, aux, nxt, nullptr, nullptr))) return rc;
+    if ((rc = csh_rhs_stage(p, nxt, CSH_RK4_4, ph, dt, w, aux, nxt, nullptr, nullptr))) return rc;
+    if ((rc = from_kymajor(w, d->stage.as<double2>(), nh, ny, 1, nullptr))) return rc;
+    TARA_HIP(hipMemcpy(omegak_new, d->stage.p, bytes, hipMemcpyDeviceToHost));
+    return TARA_OK;
+}
+
+CshPhys csh_phys(double nu)
+{
+    CshPhys ph;
+    ph.nu = nu;
+    return ph;
+}
+
+CshPhys css_phys(double alpha, double nu, double lambda)
+{
+    CshPhys ph;
+    ph.nu = nu;
+    ph.alpha = alpha;
+    ph.lambda = lambda;
+    ph.screened = 1;
+    return ph;
+}
+
+}  // namespace
+}  // namespace tara
+
+using namespace tara;
+
+extern "C" int tara_derive2csh(int nx, int ny, int nh, double nu, const double* omegak, double* dt_new)
+{
+    return derive_dropin("derive2csh", nx, ny, nh, csh_phys(nu), omegak, dt_new);
+}
+
+extern "C" int tara_derive2css(int nx, int ny, int nh, double alpha, double nu, double lambda, const double* omegak,
+                               double* dt_new)
+{
+    return derive_dropin("derive2css", nx, ny, nh, css_phys(alpha, nu, lambda), omegak, dt_new);
+}
+
+extern "C" int tara_rk42css(int nx, int ny, int nh, double alpha, double nu, double lambda, double dt,
+                            const double* omegak, double* omegak_new)
+{
+    return rk4_dropin("rk42css", nx, ny, nh, css_phys(alpha, nu, lambda), dt, omegak, omegak_new);
 }
 
 extern "C" int tara_ab2csh(int nx, int ny, int nh, double dt, const double* omegak, double* omegak_new,
@@ -141,17 +204,25 @@ extern "C" int tara_ab2csh(int nx, int ny, int nh, double dt, const double* omeg
     return TARA_OK;
 }
 
+extern "C" int tara_ab2css(int nx, int ny, int nh, double dt, const double* omegak, double* omegak_new,
+                           const double* f_old, const double* f_new)
+{
+    return tara_ab2csh(nx, ny, nh, dt, omegak, omegak_new, f_old, f_new);
+}
+
 // ------------------------------------------------------- stepping context
 struct tara_csh {
     CshPlan plan;
     DevBuf w, aux, next, stage;  // ky-major state; stage in the caller's layout
     hipStream_t st = nullptr;
-    double nu = 0, dt = 0;
+    CshPhys ph;
+    double dt = 0;
     int scheme = 0;
     std::unique_ptr<EventTimer> tm;
 };
 
-extern "C" int tara_csh_create(int nx, int ny, double nu, double dt, int scheme, tara_csh** out)
+namespace {
+int create_ctx(int nx, int ny, const CshPhys& ph, double dt, int scheme, tara_csh** out)
 {
     TARA_CHECK(out, TARA_ERR_ARG, "null out");
     TARA_CHECK(scheme == TARA_SCHEME_AB2 || scheme == TARA_SCHEME_RK4, TARA_ERR_ARG, "unknown scheme %d", scheme);
@@ -166,11 +237,23 @@ extern "C" int tara_csh_create(int nx, int ny, double nu, double dt, int scheme,
     TARA_HIP(hipMemset(c->aux.p, 0, bytes));
     if (c->next.p) TARA_HIP(hipMemset(c->next.p, 0, bytes));
     TARA_HIP(hipStreamCreate(&c->st));
-    c->nu = nu;
+    c->ph = ph;
     c->dt = dt;
     c->scheme = scheme;
     *out = c.release();
     return TARA_OK;
+}
+}  // namespace
+
+extern "C" int tara_csh_create(int nx, int ny, double nu, double dt, int scheme, tara_csh** out)
+{
+    return create_ctx(nx, ny, csh_phys(nu), dt, scheme, out);
+}
+
+extern "C" int tara_css_create(int nx, int ny, double alpha, double nu, double lambda, double dt, int scheme,
+                               tara_csh** out)
+{
+    return create_ctx(nx, ny, css_phys(alpha, nu, lambda), dt, scheme, out);
 }
 
 extern "C" int tara_csh_upload(tara_csh* c, const double* omegak)
@@ -206,13 +289,13 @@ extern "C" int tara_csh_advance(tara_csh* c, int nsteps)
     for (int s = 0; s < nsteps; ++s) {
         int rc;
         if (c->scheme == TARA_SCHEME_AB2) {
-            rc = csh_rhs_stage(c->plan, w, CSH_AB2, c->nu, c->dt, w, aux, nullptr, c->st, tm);
+            rc = csh_rhs_stage(c->plan, w, CSH_AB2, c->ph, c->dt, w, aux, nullptr, c->st, tm);
             if (rc) return rc;
         } else {
-            if ((rc = csh_rhs_stage(c->plan, w, CSH_RK4_1, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
-            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_2, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
-            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_3, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
-            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_4, c->nu, c->dt, w, aux, nx, c->st, tm))) return rc;
+            if ((rc = csh_rhs_stage(c->plan, w, CSH_RK4_1, c->ph, c->dt, w, aux, nx, c->st, tm))) return rc;
+            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_2, c->ph, c->dt, w, aux, nx, c->st, tm))) return rc;
+            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_3, c->ph, c->dt, w, aux, nx, c->st, tm))) return rc;
+            if ((rc = csh_rhs_stage(c->plan, nx, CSH_RK4_4, c->ph, c->dt, w, aux, nx, c->st, tm))) return rc;
         }
     }
     return TARA_OK;

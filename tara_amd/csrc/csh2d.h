@@ -39,11 +39,22 @@ struct CshPlan {
     const double2* rctw = nullptr;
 };
 
+// Physics of one right-hand side.  screened = 0: derive2csh (psi = omega/k^2,
+// derivatives and viscous term on omega).  screened = 1: derive2css
+// (psi = omega/(k^2 + lambda), omega_d = k^2 psi carries the derivatives and the
+// viscous term, the dealiased nonlinear term is scaled by alpha).
+struct CshPhys {
+    double nu = 0.0;
+    double alpha = 1.0;
+    double lambda = 0.0;
+    int screened = 0;
+};
+
 int csh_plan_init(CshPlan& p, int nx, int ny);
 // One right-hand-side evaluation on stage input ws followed by the fused
 // update selected by mode (see k_csh_yfwd).
-int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, double nu, double dt, double2* base, double2* aux,
-                  double2* next, hipStream_t st, EventTimer* tm);
+int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& ph, double dt, double2* base,
+                  double2* aux, double2* next, hipStream_t st, EventTimer* tm);
 int launch_ab2(const double2* w, double2* wn, const double2* fo, const double2* fn, double dt, size_t n,
                hipStream_t st);
 

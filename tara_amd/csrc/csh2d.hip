@@ -65,7 +65,7 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
                                                               const double* ky,
                                                               double2* __restrict__ g0, double2* __restrict__ g1,
                                                               double2* __restrict__ g2, double2* __restrict__ g3,
-                                                              const double2* tw)
+                                                              const double2* tw, double lam, int screened)
 {
     using S = YShape<NY>;
     constexpr int E = RF<NY>::E, T = S::T;
@@ -79,6 +79,20 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
 #pragma unroll
     for (int e = 0; e < E; ++e) wv[e] = live ? w[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
     const double kxv = (double)i;  // derive2csh's kx = i-1 exactly (checked on the host)
+    // screened (derive2css.f95:22-52): psi = omega/(k^2 + lambda) with psi(0,0) = 0,
+    // omega_d = k^2 psi; the derivatives below act on omega_d, psi is kept in ps.
+    double2 ps[E];
+    if (screened) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = rf_first_idx<NY>(t, e);
+            const double kyv = csh_ky<NY>(j);
+            const double k2 = kxv * kxv + kyv * kyv;
+            const double den = k2 + lam;
+            ps[e] = (i == 0 && j == 0) ? make_double2(0.0, 0.0) : make_double2(wv[e].x / den, wv[e].y / den);
+            wv[e] = make_double2(k2 * ps[e].x, k2 * ps[e].y);
+        }
+    }
     // i kx omega, i ky omega first; then omega -> psi in place; then +i ky psi, -i kx psi.
 #pragma unroll 1
     for (int f = 0; f < 4; ++f) {
@@ -88,7 +102,8 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
             for (int e = 0; e < E; ++e) {
                 const double kyv = csh_ky<NY>(rf_first_idx<NY>(t, e));
                 const double k2 = kxv * kxv + kyv * kyv;
-                wv[e] = (k2 != 0.0) ? make_double2(wv[e].x / k2, wv[e].y / k2) : make_double2(0.0, 0.0);
+                wv[e] = screened ? ps[e]
+                                 : (k2 != 0.0) ? make_double2(wv[e].x / k2, wv[e].y / k2) : make_double2(0.0, 0.0);
             }
         }
 #pragma unroll
@@ -216,6 +231,8 @@ struct YfwdArgs {
     const int* imin;       // dealias: modes with i >= imin[j] drop the nonlinear term
     int nh;
     double nu, dt;
+    double alpha, lam;     // screened form: alpha * NL, viscous term on k^2 ws/(k^2 + lam)
+    int screened;
 };
 
 template <int NY, int MODE>
@@ -241,9 +258,19 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const d
         double2 nl = v[e];
         if (i >= a.imin[j]) nl = make_double2(0.0, 0.0);
         const double kyv = csh_ky<NY>(j);
-        const double vk = a.nu * (kxv * kxv + kyv * kyv);
+        const double k2 = kxv * kxv + kyv * kyv;
+        const double vk = a.nu * k2;
         const double2 ws = a.ws[q];
-        const double2 f = make_double2(-(nl.x + vk * ws.x), -(nl.y + vk * ws.y));
+        double2 f;
+        if (a.screened) {
+            // derive2css.f95:118-130: -(alpha (NLx + NLy) + nu k^2 omega_d)
+            const double den = k2 + a.lam;
+            const double2 wd = (i == 0 && j == 0) ? make_double2(0.0, 0.0)
+                                                  : make_double2(k2 * (ws.x / den), k2 * (ws.y / den));
+            f = make_double2(-(a.alpha * nl.x + vk * wd.x), -(a.alpha * nl.y + vk * wd.y));
+        } else {
+            f = make_double2(-(nl.x + vk * ws.x), -(nl.y + vk * ws.y));
+        }
         if (MODE == CSH_RHS) {
             a.next[q] = f;
         } else if (MODE == CSH_AB2) {
@@ -279,12 +306,12 @@ __global__ void k_ab2(const double2* w, double2* __restrict__ wn, const double2*
 
 // ------------------------------------------------------------------ dispatch
 template <int NY>
-static int launch_yinv(const CshPlan& p, const double2* w, hipStream_t st)
+static int launch_yinv(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream_t st)
 {
     using S = YShape<NY>;
     hipLaunchKernelGGL(k_csh_yinv<NY>, dim3((p.nh + S::CB - 1) / S::CB), dim3(S::NT), 0, st, w, p.nh,
                        p.kx.as<double>(), p.ky.as<double>(), p.g[0].as<double2>(), p.g[1].as<double2>(),
-                       p.g[2].as<double2>(), p.g[3].as<double2>(), p.tw_y);
+                       p.g[2].as<double2>(), p.g[3].as<double2>(), p.tw_y, ph.lambda, ph.screened);
     return TARA_OK;
 }
 
@@ -331,7 +358,10 @@ static int launch_yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_
         default: set_error("unsupported transform length %d", n); return TARA_ERR_UNSUPPORTED; \
     }
 
-static int yinv(const CshPlan& p, const double2* w, hipStream_t st) { TARA_CSH_SWITCH(p.ny, launch_yinv, p, w, st) }
+static int yinv(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream_t st)
+{
+    TARA_CSH_SWITCH(p.ny, launch_yinv, p, w, ph, st)
+}
 static int xpass(const CshPlan& p, hipStream_t st) { TARA_CSH_SWITCH(p.nx, launch_x, p, st) }
 static int yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 {
@@ -387,12 +417,12 @@ int csh_plan_init(CshPlan& p, int nx, int ny)
     return TARA_OK;
 }
 
-int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, double nu, double dt, double2* base, double2* aux,
-                  double2* next, hipStream_t st, EventTimer* tm)
+int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& ph, double dt, double2* base,
+                  double2* aux, double2* next, hipStream_t st, EventTimer* tm)
 {
     int rc;
     if (tm) tm->begin(0, st);
-    if ((rc = yinv(p, ws, st))) return rc;
+    if ((rc = yinv(p, ws, ph, st))) return rc;
     if (tm) tm->end(0, st);
     if (tm) tm->begin(1, st);
     if ((rc = xpass(p, st))) return rc;
@@ -407,8 +437,11 @@ int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, double nu, doub
     a.ky = p.ky.as<double>();
     a.imin = p.imin.as<int>();
     a.nh = p.nh;
-    a.nu = nu;
+    a.nu = ph.nu;
     a.dt = dt;
+    a.alpha = ph.alpha;
+    a.lam = ph.lambda;
+    a.screened = ph.screened;
     if (tm) tm->begin(2, st);
     if ((rc = yfwd(p, mode, a, st))) return rc;
     if (tm) tm->end(2, st);

@@ -8,13 +8,13 @@
 //
 // Five sweeps per right-hand side (device layout: kx-fastest, row length
 // padded to ld = Nh rounded up to 8; element (i, j, k) at (k*Ny + j)*ld + i):
-//   1 k_mhd_col (z, backward)  state (7 fields) -> A, tiles of 8 kx columns
-//   2 k_mhd_col (y, backward)  A in place
+//   1 k_col_fft (z, backward)  state (7 fields) -> A, tiles of 8 kx columns
+//   2 k_col_fft (y, backward)  A in place
 //   3 k_mhd_x                  per x row: 7 c2r as 4 paired complex FFTs,
 //                              u = rho u / rho, P, B^2 and the 14 live fluxes
 //                              (derive3com.f95:176-186, 395-436), 7 paired
 //                              forward FFTs -> B (14 fields)
-//   4 k_mhd_col (y, forward)   B in place
+//   4 k_col_fft (y, forward)   B in place
 //   5 k_mhd_zfwd               z forward of the 14 fields, i k . flux, k^2
 //                              diffusion, spherical dealias on all terms
 //                              (derive3com.f95:574-843), assembly
@@ -24,45 +24,9 @@
 #include <cmath>
 
 #include "mhd3d.h"
-#include "passes.h"
-#include "rfft.h"
+#include "colpass.h"
 
 namespace tara {
-
-template <int N>
-struct CShape {
-    static constexpr int T = RF<N>::T;        // threads per column
-    static constexpr int NT = 8 * T;          // 8 kx columns per tile (128-byte rows)
-    static constexpr int LP = RF<N>::LDS + 1; // per-column buffer; +1 shifts banks between columns
-};
-
-struct ColArgs {
-    const double2* src[14];
-    double2* dst[14];
-    long es;       // element stride along the transform (complex elements)
-    long pstride;  // plane stride
-    int tpp;       // tiles (of 8 columns) per plane
-};
-
-// Column FFT over tiles of 8 adjacent kx columns; blockIdx.y selects the field.
-template <int N, int DIR>
-__global__ __launch_bounds__(CShape<N>::NT) void k_mhd_col(ColArgs a, const double2* tw)
-{
-    using S = CShape<N>;
-    constexpr int E = RF<N>::E;
-    __shared__ double2 lds[8 * S::LP];
-    const int c = threadIdx.x % 8, t = threadIdx.x / 8;  // column fastest: 8 x 16 B contiguous
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const size_t off = (size_t)(tile / a.tpp) * a.pstride + (tile % a.tpp) * 8 + c;
-    const double2* src = a.src[blockIdx.y] + off;
-    double2* dst = a.dst[blockIdx.y] + off;
-    double2 v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = src[(size_t)rf_first_idx<N>(t, e) * a.es];
-    rf_fft<N, DIR>(v, t, lds + c * S::LP, tw);
-#pragma unroll
-    for (int e = 0; e < E; ++e) dst[(size_t)rf_final_idx<N>(t, e) * a.es] = v[e];
-}
 
 // ------------------------------------------------------------------ x pass
 // One x row per wavefront group of T = NX/4 lanes (4 complex elements per
@@ -334,14 +298,18 @@ static int launch_col(const ColArgs& a, int ntiles, int nfields, int dir, const 
 {
     dim3 g(ntiles, nfields), b(CShape<N>::NT);
     if (dir < 0)
-        hipLaunchKernelGGL((k_mhd_col<N, -1>), g, b, 0, st, a, tw);
+        hipLaunchKernelGGL((k_col_fft<N, -1>), g, b, 0, st, a, tw);
     else
-        hipLaunchKernelGGL((k_mhd_col<N, +1>), g, b, 0, st, a, tw);
+        hipLaunchKernelGGL((k_col_fft<N, +1>), g, b, 0, st, a, tw);
     return TARA_OK;
+}
+int col_fft_tiles(int n, const ColArgs& a, int ntiles, int nfields, int dir, const double2* tw, hipStream_t st)
+{
+    TARA_MHD_SWITCH(n, launch_col, a, ntiles, nfields, dir, tw, st)
 }
 static int col(int n, const ColArgs& a, int ntiles, int nfields, int dir, const double2* tw, hipStream_t st)
 {
-    TARA_MHD_SWITCH(n, launch_col, a, ntiles, nfields, dir, tw, st)
+    return col_fft_tiles(n, a, ntiles, nfields, dir, tw, st);
 }
 
 template <int NX>

@@ -21,6 +21,12 @@ def golden():
         return {k: z[k] for k in z.files}
 
 
+@pytest.fixture(scope="session")
+def golden_fam2d():
+    with np.load(os.path.join(ROOT, "tests", "golden", "golden_fam2d.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
 def rel_l2(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

@@ -7,7 +7,7 @@ are produced by running its routines on the inputs below.  Run here (where
 /root/reference exists); the .npz files travel with the repository, the
 reference does not.
 
-    ulimit -s unlimited && python tests/golden/make_golden.py
+    ulimit -s unlimited && python tests/golden/make_golden.py [core|fam2d|all]
 
 (derive3com keeps ~130 automatic arrays on the stack, hence the stack limit.)
 """
@@ -48,6 +48,87 @@ def mhd_state(seed, n):
 def random_spec(seed, n):
     rng = np.random.default_rng(seed)
     return O.r2c(np.asfortranarray(rng.standard_normal((n, n))))
+
+
+def coh_state(seed, n):
+    """n = 1 + small perturbation, smooth ux, uy (Lx = Ly = 2 pi)."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * 2 * PI / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+
+    def smooth():
+        f = np.zeros((n, n))
+        for _ in range(5):
+            k = rng.integers(-3, 4, size=2)
+            f += rng.standard_normal() * np.cos(k[0] * X + k[1] * Y + rng.uniform(0, 2 * PI))
+        return f
+
+    dens = 1.0 + 0.05 * smooth()
+    return X, Y, dens, 0.2 * smooth(), 0.2 * smooth(), smooth
+
+
+def fam2d(out):
+    """2D compressible families from the reference build: derive2coh, derive2com
+    (forcing off: A = kf = 0, its x, y arrays being uninitialised), the coh and com
+    AB2 loops (src/main2coh.f95:278-300, src/main2com.f95:364-380; history zero),
+    and the pointwise steppers."""
+    pi, L = PI, 2 * PI
+    coh = dict(pi=pi, lx=L, ly=L, mux=2e-4, muy=2e-4, cs=3.77)
+    com = dict(pi=pi, lx=L, ly=L, mu=1e-3, eta=1e-4, cs=2.0)
+    for n in (32, 64):
+        _, _, dens, ux, uy, smooth = coh_state(200 + n, n)
+        st = [O.r2c(np.asfortranarray(a)) for a in (dens, ux, uy)]
+        f = O.ref_derive2coh(st, n, **coh)
+        for k in range(3):
+            out[f"coh_{n}_in{k}"] = st[k]
+            out[f"coh_{n}_out{k}"] = f[k]
+        rho = dens
+        E = 2.0 + 0.1 * smooth()
+        bx, by = 0.1 * smooth(), 0.1 * smooth()
+        st = [O.r2c(np.asfortranarray(a)) for a in (rho, rho * ux, rho * uy, E, bx, by)]
+        f = O.ref_derive2com(st, n, **com)
+        assert all(np.isfinite(x).all() for x in f)
+        for k in range(6):
+            out[f"com_{n}_in{k}"] = st[k]
+            out[f"com_{n}_out{k}"] = f[k]
+    # 200-step AB2 loops at 32^2
+    n, dt = 32, 1e-4
+    st = [out[f"coh_{n}_in{k}"] for k in range(3)]
+    hist = [np.zeros_like(st[0], order="F") for _ in range(3)]
+    for _ in range(200):
+        f = O.ref_derive2coh(st, n, **coh)
+        st = O.ref_ab2coh(st, hist, f, n, dt)
+        hist = f
+    for k in range(3):
+        out[f"cohtraj_{n}_w{k}"] = st[k]
+    st = [out[f"com_{n}_in{k}"] for k in range(6)]
+    hist = [np.zeros_like(st[0], order="F") for _ in range(6)]
+    for _ in range(200):
+        f = O.ref_derive2com(st, n, **com)
+        st = O.ref_ab2com(st, hist, f, n, dt)
+        hist = f
+    for k in range(6):
+        out[f"comtraj_{n}_w{k}"] = st[k]
+    # pointwise steppers at 32^2
+    st = [out[f"coh_{n}_in{k}"] for k in range(3)]
+    sl = [random_spec(300 + k, n) for k in range(3)]
+    fo = [random_spec(310 + k, n) for k in range(3)]
+    for k in range(3):
+        out[f"pt_slope{k}"] = sl[k]
+        out[f"pt_fold{k}"] = fo[k]
+    for k, a in enumerate(O.ref_ab2coh(st, fo, sl, n, 1e-3)):
+        out[f"pt_ab2coh{k}"] = a
+    for k, a in enumerate(O.ref_euler2coh(st, sl, n, 1e-3)):
+        out[f"pt_euler{k}"] = a
+    out["pt_rk4nk"] = O.ref_nk_stepper("rk4_nk2coh_", st, sl, n, 1e-3, **coh)
+    out["pt_pcnk"] = O.ref_nk_stepper("pc_nk2coh_", st, sl, n, 1e-3, **coh)
+
+
+def main_fam2d():
+    out = {}
+    fam2d(out)
+    np.savez_compressed(os.path.join(HERE, "golden_fam2d.npz"), **out)
+    print("wrote", len(out), "arrays to golden_fam2d.npz")
 
 
 def main():
@@ -103,4 +184,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("core", "all"):
+        main()
+    if what in ("fam2d", "all"):
+        main_fam2d()

@@ -210,3 +210,115 @@ def test_derive3com_induction_divergence_free(golden):
     div = K[0] * f[5] + K[1] * f[6] + K[2] * f[7]
     scale = np.sqrt(sum(np.sum(np.abs(K[c] * f[5 + c]) ** 2) for c in range(3)))
     assert np.linalg.norm(div) / scale < 1e-13
+
+
+# ------------------------------------------------------------------ derive2css
+def _cos_field(n):
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    return X, Y, np.cos(X) + np.cos(2 * Y)
+
+
+@pytest.mark.parametrize("lam,alpha,nu", [(10.0, 1.0, 0.0), (10.0, 0.7, 0.01), (2.5, 1.3, 0.1)])
+def test_derive2css_closed_form(lam, alpha, nu):
+    """omega = cos x + cos 2y: psi = cos x/(1+lam) + cos 2y/(4+lam), omega_d = k^2 psi,
+    u . grad omega_d = -6 sin x sin 2y / ((1+lam)(4+lam)), so
+    f = 6 alpha sin x sin 2y/((1+lam)(4+lam)) - nu cos x/(1+lam) - 16 nu cos 2y/(4+lam)."""
+    n = 32
+    X, Y, w = _cos_field(n)
+    f = O.c2r(O.derive2css(O.r2c(w), n, alpha, nu, lam), n) / (n * n)
+    exact = (6 * alpha * np.sin(X) * np.sin(2 * Y) / ((1 + lam) * (4 + lam))
+             - nu * np.cos(X) / (1 + lam) - 16 * nu * np.cos(2 * Y) / (4 + lam))
+    assert np.max(np.abs(f - exact)) < 1e-13
+
+
+def test_derive2css_unscreened_is_reference_derive2csh(golden):
+    """lambda = 0, alpha = 1 reduces derive2css to derive2csh; checked against the
+    reference build's derive2csh vectors."""
+    for n in (64, 128):
+        w = golden[f"d2csh_{n}_rnd_in"]
+        assert rel_l2(O.derive2css(w, n, 1.0, 1e-3, 0.0), golden[f"d2csh_{n}_rnd_out"]) < 1e-13
+
+
+def test_derive2css_mask_count():
+    """The circular dealias acts on the nonlinear part only: with nu = 0 the
+    RHS vanishes on exactly the modes sqrt(kx^2+ky^2) >= (Nx+Ny)/6."""
+    n = 64
+    rng = np.random.default_rng(5)
+    w = O.r2c(rng.standard_normal((n, n)))
+    f = O.derive2css(w, n, 1.0, 0.0, 10.0)
+    kx = np.arange(n // 2 + 1)[:, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :]
+    cut = np.sqrt(kx * kx + ky * ky) >= (2 * n / 2.0) / 3.0
+    assert int(cut.sum()) == 1374
+    assert np.all(f[cut] == 0) and np.all(f[~cut][1:] != 0)
+
+
+def test_rk4css_fourth_order():
+    """Linear screened decay (single mode, nu > 0, no nonlinearity): the exact
+    solution is w0 exp(-nu k^4/(k^2+lam) t); RK4's error falls ~16x per halving."""
+    n, lam, nu = 16, 10.0, 0.5
+    w = np.zeros((n // 2 + 1, n), np.complex128, order="F")
+    w[3, 0] = 1.0 + 0.5j
+    k2 = 9.0
+    rate = nu * k2 * k2 / (k2 + lam)
+    errs = []
+    for steps in (10, 20, 40):
+        got = O.rk4css_steps(w, n, 1.0, nu, lam, 0.4 / steps, steps)
+        errs.append(abs(got[3, 0] - w[3, 0] * np.exp(-rate * 0.4)))
+    assert 14 < errs[0] / errs[1] < 18 and 14 < errs[1] / errs[2] < 18
+
+
+# ------------------------------------------------- 2D compressible families
+COH = dict(pi=np.pi, lx=2 * np.pi, ly=2 * np.pi, mux=2e-4, muy=2e-4, cs=3.77)
+COM = dict(pi=np.pi, lx=2 * np.pi, ly=2 * np.pi, mu=1e-3, eta=1e-4, cs=2.0)
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_derive2coh_vs_reference(golden_fam2d, n):
+    st = [golden_fam2d[f"coh_{n}_in{k}"] for k in range(3)]
+    f = O.derive2coh(st, n, **COH)
+    for k in range(3):
+        assert rel_l2(f[k], golden_fam2d[f"coh_{n}_out{k}"]) < 1e-13
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_derive2com_vs_reference(golden_fam2d, n):
+    st = [golden_fam2d[f"com_{n}_in{k}"] for k in range(6)]
+    f = O.derive2com(st, n, kf=0.0, a=0.0, **COM)
+    for k in range(6):
+        assert rel_l2(f[k], golden_fam2d[f"com_{n}_out{k}"]) < 1e-13
+
+
+def test_derive2coh_uniform_state_is_steady():
+    n = 32
+    st = [O.r2c(np.ones((n, n))), O.r2c(np.zeros((n, n))), O.r2c(np.zeros((n, n)))]
+    for f in O.derive2coh(st, n, **COH):
+        assert np.max(np.abs(f)) < 1e-12
+
+
+def test_derive2com_forcing_closed_form():
+    """rho = 1, u = 0, B = 0, E uniform: every flux is uniform, so the momentum
+    RHS is the (undealiased) forcing spectrum alone: F = A (-sin(kf y), sin(kf x))
+    on the grid x(i) = (i-1) dx, y(j) = (j-1) dy; the other RHS vanish."""
+    n, A = 32, 0.3
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    st = [O.r2c(a) for a in (np.ones((n, n)), np.zeros((n, n)), np.zeros((n, n)), 2 * np.ones((n, n)),
+                             np.zeros((n, n)), np.zeros((n, n)))]
+    f = O.derive2com(st, n, kf=1.0, a=A, **COM)
+    assert rel_l2(f[1], O.r2c(-A * np.sin(Y))) < 1e-14
+    assert rel_l2(f[2], O.r2c(A * np.sin(X))) < 1e-14
+    for k in (0, 3, 4, 5):
+        assert np.max(np.abs(f[k])) < 1e-12
+
+
+def test_coh_steppers_closed_forms(golden_fam2d):
+    """The reference's rk4_nk2coh and pc_nk2coh reduce to closed forms of the
+    input slope (their derive2coh calls write unused locals): pinned bit for bit
+    against the reference build."""
+    s = golden_fam2d["pt_slope0"]
+    w = golden_fam2d["coh_32_in0"]
+    dt = 1e-3
+    assert np.array_equal(golden_fam2d["pt_rk4nk"], w + (dt / 6.0) * (((s + 2.0 * s) + 2.0 * s) + s))
+    assert np.array_equal(golden_fam2d["pt_pcnk"], w + ((s + s) * dt) / 2.0)
