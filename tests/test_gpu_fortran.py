@@ -1,7 +1,8 @@
-"""The Fortran boundary: the reference's routine names (derive2csh, ab2csh)
-resolved to tara_amd/fortran/csh_dropin.f90 -> C ABI -> HIP, driven by a
-Fortran time loop (tests/fortran/drive_csh.f90, the loop of
-src/main2csh.f95:114-126), against the reference trajectory."""
+"""The Fortran boundary: the reference's routine names (derive2csh, ab2csh,
+derive2coh, ab2coh, derive2com, ab2com, derive3com, ab3com, ...) resolved to
+tara_amd/fortran/*_dropin.f90 -> C ABI -> HIP, driven by Fortran time loops
+(tests/fortran/drive_*.f90, the loops of the reference's main programs, with
+their COMMON /comm/ layouts), against the reference build's trajectories."""
 import os
 import subprocess
 
@@ -27,11 +28,39 @@ def _read(path):
     return np.asfortranarray(w.reshape(ny, nx // 2 + 1).T)
 
 
+REFERENCE_NAMES = ("derive2csh", "ab2csh", "derive2css", "ab2css", "rk42css", "derive2coh", "ab2coh", "euler2coh",
+                   "rk4_nk2coh", "rk4_ukx2coh", "rk4_uky2coh", "pc_nk2coh", "pc_ukx2coh", "pc_uky2coh",
+                   "derive2com", "ab2com", "derive3com", "ab3com")
+
+
 def test_fortran_driver_exports():
     out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "tara_amd", "libtara_fortran.so")],
                          capture_output=True, text=True).stdout
-    for sym in ("derive2csh_", "ab2csh_"):
-        assert f" T {sym}" in out
+    for sym in REFERENCE_NAMES:
+        assert f" T {sym}_" in out, sym
+
+
+def _write_fields(path, dims, fields):
+    with open(path, "wb") as f:
+        np.array(list(dims) + [len(fields)], dtype="<i4").tofile(f)
+        for a in fields:
+            np.asfortranarray(a).T.astype("<c16").tofile(f)
+
+
+def _read_fields(path, ndim):
+    with open(path, "rb") as f:
+        dims = np.fromfile(f, dtype="<i4", count=ndim)
+        nf = int(np.fromfile(f, dtype="<i4", count=1)[0])
+        shape = (dims[0] // 2 + 1,) + tuple(int(d) for d in dims[1:])
+        n = int(np.prod(shape))
+        return [np.asfortranarray(np.fromfile(f, dtype="<c16", count=n).reshape(shape[::-1]).T) for _ in range(nf)]
+
+
+def test_fields_io_roundtrip(tmp_path):
+    a = [np.asfortranarray(np.arange(9 * 16 * 4).reshape(9, 16, 4) * (1 + 2j) + k) for k in range(3)]
+    _write_fields(tmp_path / "a.bin", (16, 16, 4), a)
+    b = _read_fields(tmp_path / "a.bin", 3)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
 def test_io_roundtrip(tmp_path):
@@ -49,3 +78,51 @@ def test_fortran_dropin_config1(golden, tmp_path, tag):
     subprocess.run([DRIVER, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "2000", "0.01", "1.0d-4"],
                    check=True, timeout=120)
     assert rel_l2(_read(tmp_path / "out.bin"), golden[f"c1_{tag}_w2000"]) < 1e-10
+
+
+def _driver(name):
+    exe = os.path.join(ROOT, "tests", "fortran", name)
+    assert os.path.exists(exe), f"tests/fortran/{name} not built (run __graft_entry__.build())"
+    return exe
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_coh(golden_fam2d, tmp_path):
+    n = 32
+    _write_fields(tmp_path / "in.bin", (n, n), [golden_fam2d[f"coh_{n}_in{k}"] for k in range(3)])
+    print("fortran coh", flush=True)
+    subprocess.run([_driver("drive_coh"), str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "200", "1.0d-4",
+                    "2.0d-4", "3.77d0"], check=True, timeout=120)
+    got = _read_fields(tmp_path / "out.bin", 2)
+    for k in range(3):
+        assert rel_l2(got[k], golden_fam2d[f"cohtraj_{n}_w{k}"]) < 1e-10
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_com(golden_fam2d, tmp_path):
+    n = 32
+    _write_fields(tmp_path / "in.bin", (n, n), [golden_fam2d[f"com_{n}_in{k}"] for k in range(6)])
+    print("fortran com", flush=True)
+    subprocess.run([_driver("drive_com"), str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "200", "1.0d-4",
+                    "1.0d-3", "1.0d-4", "2.0d0"], check=True, timeout=120)
+    got = _read_fields(tmp_path / "out.bin", 2)
+    for k in range(6):
+        assert rel_l2(got[k], golden_fam2d[f"comtraj_{n}_w{k}"]) < 1e-10
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_mhd_one_step(golden, tmp_path):
+    """One derive3com + ab3com step through the Fortran names with the
+    reference's 1/450 diffusion: from zero history the update is
+    state + 3/2 dt f, f = the reference derive3com vector; E is unchanged."""
+    n, dt = 16, 1e-3
+    st = [golden[f"d3com_in{k}"] for k in range(8)]
+    _write_fields(tmp_path / "in.bin", (n, n, n), st)
+    print("fortran mhd", flush=True)
+    subprocess.run([_driver("drive_mhd"), str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "1", "1.0d-3",
+                    "10.0d0"], check=True, timeout=120)
+    got = _read_fields(tmp_path / "out.bin", 3)
+    assert np.array_equal(got[4], st[4])
+    for k in (0, 1, 2, 3, 5, 6, 7):
+        # the increment, so that the state's size does not hide an error in f
+        assert rel_l2(got[k] - st[k], 1.5 * dt * golden[f"d3com_out{k}"]) < 1e-12, k
