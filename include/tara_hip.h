@@ -78,6 +78,14 @@ int tara_csh_destroy(tara_csh* ctx);
  * timed launches and their summed duration in milliseconds, then resets. */
 int tara_csh_set_timing(tara_csh* ctx, int on);
 int tara_csh_kernel_stats(tara_csh* ctx, int nclass, long* counts, double* ms);
+/* Autotuner (no counterpart in the reference, whose FFTW planner runs in
+ * FFTW_ESTIMATE mode): times every x-pass shape that runs on this grid (reps
+ * launches each, on the context's stream) and keeps the fastest.  ms, if not
+ * NULL, receives per-shape milliseconds (-1 for shapes that do not run);
+ * tara_csh_num_xshapes() gives its length.  tara_csh_set_xshape forces one. */
+int tara_csh_num_xshapes(void);
+int tara_csh_tune(tara_csh* ctx, int reps, int* chosen, double* ms);
+int tara_csh_set_xshape(tara_csh* ctx, int shape);
 
 /* ------------------------------------- 2D screened vorticity (css family)
  * Replaces src/derive2css.f95:6 derive2css(Nx,Ny,Nh,alpha,nu,time,omegak,

@@ -80,6 +80,9 @@ def lib():
         L.tara_csh_destroy.argtypes = [C.c_void_p]
         L.tara_csh_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.tara_csh_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
+        L.tara_csh_num_xshapes.argtypes = []
+        L.tara_csh_tune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]
+        L.tara_csh_set_xshape.argtypes = [C.c_void_p, C.c_int]
         P3, P6 = C.c_void_p * 3, C.c_void_p * 6
         L.tara_derive2coh.argtypes = [C.c_int] * 3 + [C.POINTER(CohParams), P3, P3]
         L.tara_ab2coh.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, P3, P3]
@@ -243,6 +246,18 @@ class CshSolver:
         ms = (C.c_double * nclass)()
         _check(lib().tara_csh_kernel_stats(self._h, nclass, counts, ms))
         return list(counts), list(ms)
+
+    def tune(self, reps=5):
+        """Times every x-pass shape on this grid, keeps the fastest; returns
+        (chosen shape, per-shape ms with -1 for shapes that do not run)."""
+        n = lib().tara_csh_num_xshapes()
+        ms = (C.c_double * n)()
+        chosen = C.c_int()
+        _check(lib().tara_csh_tune(self._h, reps, C.byref(chosen), ms))
+        return chosen.value, list(ms)
+
+    def set_xshape(self, shape):
+        _check(lib().tara_csh_set_xshape(self._h, shape))
 
     def close(self):
         if self._h:

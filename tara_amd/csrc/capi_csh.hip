@@ -320,6 +320,60 @@ extern "C" int tara_csh_destroy(tara_csh* c)
     return TARA_OK;
 }
 
+extern "C" int tara_csh_num_xshapes(void) { return csh_num_xshapes(); }
+
+extern "C" int tara_csh_tune(tara_csh* c, int reps, int* chosen, double* ms)
+{
+    TARA_CHECK(c && reps > 0, TARA_ERR_ARG, "bad argument");
+    const int n = csh_num_xshapes();
+    hipEvent_t e0, e1;
+    TARA_HIP(hipEventCreate(&e0));
+    TARA_HIP(hipEventCreate(&e1));
+    const int keep = c->plan.xcfg;
+    int best = -1;
+    float best_ms = 0.f;
+    int rc = TARA_OK;
+    for (int k = 0; k < n && rc == TARA_OK; ++k) {
+        if (ms) ms[k] = -1.0;
+        if (!csh_xshape_ok(c->plan, k)) continue;
+        c->plan.xcfg = k;
+        if ((rc = csh_xpass(c->plan, c->st))) break;  // warm-up
+        if (hipEventRecord(e0, c->st) != hipSuccess) { rc = TARA_ERR_HIP; break; }
+        for (int r = 0; r < reps && rc == TARA_OK; ++r) rc = csh_xpass(c->plan, c->st);
+        if (rc) break;
+        float t = 0.f;
+        if (hipEventRecord(e1, c->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            rc = TARA_ERR_HIP;
+            break;
+        }
+        t /= (float)reps;
+        if (ms) ms[k] = t;
+        if (best < 0 || t < best_ms) {
+            best = k;
+            best_ms = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        c->plan.xcfg = keep;
+        return rc;
+    }
+    c->plan.xcfg = best >= 0 ? best : keep;
+    if (chosen) *chosen = c->plan.xcfg;
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_set_xshape(tara_csh* c, int shape)
+{
+    TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    TARA_CHECK(csh_xshape_ok(c->plan, shape), TARA_ERR_UNSUPPORTED, "x pass shape %d does not run on %dx%d", shape,
+               c->plan.nx, c->plan.ny);
+    c->plan.xcfg = shape;
+    return TARA_OK;
+}
+
 extern "C" int tara_csh_set_timing(tara_csh* c, int on)
 {
     TARA_CHECK(c, TARA_ERR_ARG, "null context");

@@ -92,7 +92,7 @@ std::mutex g_tab_mu;
 std::map<std::tuple<int, int, int>, void*> g_tabs;  // (device, kind, n) -> device ptr
 
 // kind: 0 = Stockham table with radix cap 8, 1 = real<->complex packing roots,
-// 4 = Stockham table with radix cap 4.
+// 4 = Stockham table with radix cap 4, 16 = Stockham table with radix cap 16.
 const double2* cached_table(int kind, int n)
 {
     int dev = current_device();
@@ -104,7 +104,7 @@ const double2* cached_table(int kind, int n)
     if (kind == 1)
         host_rc_twiddles(n, h);
     else
-        host_twiddles(n, kind == 4 ? 4 : 8, h);
+        host_twiddles(n, kind == 4 ? 4 : kind == 16 ? 16 : 8, h);
     void* d = nullptr;
     if (hipMalloc(&d, h.size() * sizeof(double2)) != hipSuccess) return nullptr;
     if (hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
@@ -116,7 +116,7 @@ const double2* cached_table(int kind, int n)
 }
 }  // namespace
 
-const double2* twiddles(int n, int rmax) { return cached_table(rmax == 4 ? 4 : 0, n); }
+const double2* twiddles(int n, int rmax) { return cached_table(rmax == 4 ? 4 : rmax == 16 ? 16 : 0, n); }
 const double2* rc_twiddles(int n) { return cached_table(1, n); }
 
 }  // namespace tara

@@ -36,6 +36,9 @@ struct CshPlan {
     const double2* tw_y = nullptr;
     const double2* tw_x = nullptr;   // length nx/2 (forward half-length x transform)
     const double2* tw_xn = nullptr;  // length nx (paired backward x transforms)
+    const double2* tw_x16 = nullptr;   // the same two tables for radix-16 schedules
+    const double2* tw_xn16 = nullptr;
+    int xcfg = 0;                    // x-pass shape (see csh_xshape_ok / the autotuner)
     const double2* rctw = nullptr;
 };
 
@@ -52,9 +55,16 @@ struct CshPhys {
 
 int csh_plan_init(CshPlan& p, int nx, int ny);
 // One right-hand-side evaluation on stage input ws followed by the fused
-// update selected by mode (see k_csh_yfwd).
+// update selected by mode (see k_csh_yfwd).  Timing classes: 0 inverse y,
+// 1 x pass, 2 forward y + update.
 int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& ph, double dt, double2* base,
                   double2* aux, double2* next, hipStream_t st, EventTimer* tm);
+// x-pass shapes: the default for nx, how many exist, whether shape c runs on
+// this plan's grid, and the x pass alone (for timing by the autotuner).
+int csh_default_xshape(int nx);
+int csh_num_xshapes();
+int csh_xshape_ok(const CshPlan& p, int c);
+int csh_xpass(const CshPlan& p, hipStream_t st);
 int launch_ab2(const double2* w, double2* wn, const double2* fo, const double2* fn, double dt, size_t n,
                hipStream_t st);
 

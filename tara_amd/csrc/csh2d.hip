@@ -4,12 +4,12 @@
 //
 // Device layout: the state half spectra are stored ky-major ("transposed"),
 // element (kx = i, ky = j) at i*Ny + j, so a kx column (one y transform) is a
-// contiguous Ny-element run.  The intermediates G0..G3 and H are stored in
-// 8-row blocks, element (i, j) at ((j/8)*Nh + i)*8 + j%8 (blk8): a y pass reads
-// or writes a column as whole 128-byte lines, and the x pass (RB = 2 rows per
-// workgroup at Nx = 4096) reads 32-byte pieces out of one contiguous
-// 8-row group, whose other pieces belong to the neighbouring workgroups that the
-// XCD-aware mapping runs next to it.  The reference's (Nh, Ny) kx-fastest arrays
+// contiguous Ny-element run.  The intermediates are stored in row groups
+// interleaved per kx: G0..G3 in 4-row groups (blkg), H in 8-row groups (blk8,
+// element (i, j) at ((j/8)*Nh + i)*8 + j%8): a y pass reads or writes a column
+// as 64- or 128-byte runs, and the x pass (RB = 2 rows per workgroup at
+// Nx = 4096) reads 32-byte pieces of those runs, whose other pieces belong to the
+// neighbouring workgroups that the XCD-aware mapping runs next to it.  The reference's (Nh, Ny) kx-fastest arrays
 // are converted on upload / download (layout.hip).
 //
 //   pass 1  k_csh_yinv : per kx column read omega_hat, form the four spectra of
@@ -51,6 +51,16 @@ __device__ __forceinline__ size_t blk8(int i, int j, int nh)
     return ((size_t)(j >> 3) * nh + i) * 8 + (j & 7);
 }
 
+// Offset of element (i, j) in the intermediates G0..G3: 4-row groups
+// interleaved per kx.  The x pass (2 rows per workgroup at Nx = 4096) then uses
+// half of every 64-byte run it touches and the y pass writes 64-byte runs; at
+// 4096^2 this measured faster than 8-row (yinv 0.20 ms, x 0.39 ms), 2-row
+// (0.25, 0.31) and plain row-major (0.34, 0.31) groups (yinv 0.215, x 0.33).
+__device__ __forceinline__ size_t blkg(int i, int j, int nh)
+{
+    return ((size_t)(j >> 2) * nh + i) * 4 + (j & 3);
+}
+
 // derive2csh's ky: j-1 for j <= Ny/2, j-1-Ny above (exact integers, checked on the host).
 template <int NY>
 __device__ __forceinline__ double csh_ky(int j)
@@ -59,28 +69,21 @@ __device__ __forceinline__ double csh_ky(int j)
 }
 
 // --------------------------------------------------------------------- pass 1
+// The inverse y pass of one kx column i given omega_hat (wv, at rf_first_idx
+// positions): forms the four spectra of derive2csh.f95:29-71 (i kx omega,
+// i ky omega, i ky psi, -i kx psi; psi = omega/k^2, psi(0,0) = 0), or of
+// derive2css.f95:22-52 when screened, runs the backward y FFTs and writes
+// G0..G3.  x is the column's LDS exchange buffer.
 template <int NY>
-__global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, int nh,
-                                                              const double* kx,
-                                                              const double* ky,
-                                                              double2* __restrict__ g0, double2* __restrict__ g1,
-                                                              double2* __restrict__ g2, double2* __restrict__ g3,
-                                                              const double2* tw, double lam, int screened)
+__device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i, bool live, int t, double2* x,
+                                                const double2* tw, double lam, int screened, int nh,
+                                                double2* __restrict__ g0, double2* __restrict__ g1,
+                                                double2* __restrict__ g2, double2* __restrict__ g3)
 {
-    using S = YShape<NY>;
-    constexpr int E = RF<NY>::E, T = S::T;
-    __shared__ double2 lds[S::CB * RF<NY>::LDS];
-    const int c = threadIdx.x / T, t = threadIdx.x % T;
-    const int i = xcd_remap(blockIdx.x, gridDim.x) * S::CB + c;
-    const bool live = i < nh;
-    double2* x = lds + c * RF<NY>::LDS;
-    const size_t col = (size_t)(live ? i : 0) * NY;
-    double2 wv[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) wv[e] = live ? w[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
+    constexpr int E = RF<NY>::E;
     const double kxv = (double)i;  // derive2csh's kx = i-1 exactly (checked on the host)
-    // screened (derive2css.f95:22-52): psi = omega/(k^2 + lambda) with psi(0,0) = 0,
-    // omega_d = k^2 psi; the derivatives below act on omega_d, psi is kept in ps.
+    // screened: psi = omega/(k^2 + lambda) with psi(0,0) = 0, omega_d = k^2 psi;
+    // the derivatives below act on omega_d, psi is kept in ps.
     double2 ps[E];
     if (screened) {
 #pragma unroll
@@ -119,9 +122,28 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
         double2* o = f == 0 ? g2 : f == 1 ? g3 : f == 2 ? g0 : g1;
         if (live) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) o[blk8(i, rf_final_idx<NY>(t, e), nh)] = v[e];
+            for (int e = 0; e < E; ++e) o[blkg(i, rf_final_idx<NY>(t, e), nh)] = v[e];
         }
     }
+}
+
+template <int NY>
+__global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, int nh,
+                                                              double2* __restrict__ g0, double2* __restrict__ g1,
+                                                              double2* __restrict__ g2, double2* __restrict__ g3,
+                                                              const double2* tw, double lam, int screened)
+{
+    using S = YShape<NY>;
+    constexpr int E = RF<NY>::E, T = S::T;
+    __shared__ double2 lds[S::CB * RF<NY>::LDS];
+    const int c = threadIdx.x / T, t = threadIdx.x % T;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * S::CB + c;
+    const bool live = i < nh;
+    const size_t col = (size_t)(live ? i : 0) * NY;
+    double2 wv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) wv[e] = live ? w[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
+    csh_yinv_column<NY>(wv, i, live, t, lds + c * RF<NY>::LDS, tw, lam, screened, nh, g0, g1, g2, g3);
 }
 
 // --------------------------------------------------------------------- pass 2
@@ -130,16 +152,16 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
 // length Nx (X[Nx-k] = conj X[k]) transforms to y = x_a + i x_b, so the product
 // x_a * x_b is Re(y) Im(y).  The imaginary parts of X[0] and X[M] are dropped,
 // as FFTW's c2r does.  Registers end with y at rf_final_idx<NX> positions.
-template <int NX>
+template <int NX, int EM, bool SPLIT>
 __device__ __forceinline__ void x_c2r_pair(const double2* ga, const double2* gb, int ny, int j, int t,
-                                           double2* xl, const double2* tw, double2 (&v)[RF<NX>::E])
+                                           double2* xl, const double2* tw, double2 (&v)[RF<NX, EM>::E])
 {
     constexpr int M = NX / 2;
 #pragma unroll
-    for (int e = 0; e < RF<NX>::E; ++e) {
-        const int k = rf_first_idx<NX>(t, e);
+    for (int e = 0; e < RF<NX, EM>::E; ++e) {
+        const int k = rf_first_idx<NX, EM>(t, e);
         const int kk = k <= M ? k : NX - k;
-        double2 a = ga[blk8(kk, j, M + 1)], b = gb[blk8(kk, j, M + 1)];
+        double2 a = ga[blkg(kk, j, M + 1)], b = gb[blkg(kk, j, M + 1)];
         if (kk == 0 || kk == M) {
             a.y = 0.0;
             b.y = 0.0;
@@ -150,42 +172,44 @@ __device__ __forceinline__ void x_c2r_pair(const double2* ga, const double2* gb,
         }
         v[e] = make_double2(a.x - b.y, a.y + b.x);
     }
-    rf_fft<NX, +1>(v, t, xl, tw);
+    rf_fft<NX, +1, EM, false, SPLIT>(v, t, xl, tw);
 }
 
-template <int NX>
+// Shape of the x pass: EM complex elements per thread (8: radix-8 passes; 16:
+// radix-16), RB rows per workgroup, SPLIT: the FFT exchanges go through LDS as
+// doubles in two rounds (half the LDS, so that two workgroups share a CU).
+template <int NX, int EM, int RB, bool SPLIT>
 struct XShape {
     static constexpr int M = NX / 2;
-    static constexpr int T = RF<NX>::T;    // threads per row (backward transforms, length NX)
-    static constexpr int TM = RF<M>::T;    // of which active in the forward half-length transform
-    static constexpr int RB = (1024 / T) < 1 ? 1 : (1024 / T) > 8 ? 8 : (1024 / T);
+    static constexpr int T = RF<NX, EM>::T;    // threads per row (backward transforms, length NX)
+    static constexpr int TM = RF<M, EM>::T;    // of which active in the forward half-length transform
     static constexpr int NT = RB * T;
     // Per-row exchange buffer (complex elements).  Rows are interleaved across
     // lanes, so successive row buffers are offset by 4 elements (64 B) to land
     // on different LDS banks.
-    static constexpr int LDSR = RF<NX>::LDS + 4;
-    static_assert(RB * LDSR * 16 <= 160 * 1024, "x pass exchange buffer exceeds LDS");
-    static_assert(2 * TM == T, "forward transform uses half of the row's threads");
+    static constexpr int LDSR = (SPLIT ? RF<NX, EM>::LDS / 2 : RF<NX, EM>::LDS) + 4;
+    // the forward half-length transform must use exactly half of the row's threads
+    static constexpr bool OK = RB * LDSR * 16 <= 160 * 1024 && NT <= 1024 && NT >= 64 && 2 * TM == T;
 };
 
-template <int NX>
-__global__ __launch_bounds__(XShape<NX>::NT) void k_csh_x(const double2* g0, const double2* g1, const double2* g2,
-                                                           const double2* g3, int ny, double scale,
-                                                           double2* __restrict__ h, const double2* tw_n,
-                                                           const double2* tw_m, const double2* rctw)
+template <int NX, int EM, int RB, bool SPLIT>
+__global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
+    const double2* g0, const double2* g1, const double2* g2, const double2* g3, int ny, double scale,
+    double2* __restrict__ h, const double2* tw_n, const double2* tw_m, const double2* rctw)
 {
-    using S = XShape<NX>;
-    constexpr int E = RF<NX>::E, T = S::T, M = S::M, RB = S::RB;
+    using S = XShape<NX, EM, RB, SPLIT>;
+    static_assert(S::OK, "x pass shape does not fit");
+    constexpr int E = RF<NX, EM>::E, EH = RF<S::M, EM>::E, T = S::T, M = S::M;
     __shared__ double2 lds[RB * S::LDSR];
     const int r = threadIdx.x % RB, t = threadIdx.x / RB;  // row fastest: RB-row chunks per kx
     const int j = xcd_remap(blockIdx.x, gridDim.x) * RB + r;
     double2* xl = lds + r * S::LDSR;
     double nl[E];
     double2 v[E];
-    x_c2r_pair<NX>(g0, g2, ny, j, t, xl, tw_n, v);   // ux + i d omega/dx
+    x_c2r_pair<NX, EM, SPLIT>(g0, g2, ny, j, t, xl, tw_n, v);   // ux + i d omega/dx
 #pragma unroll
     for (int e = 0; e < E; ++e) nl[e] = v[e].x * v[e].y;
-    x_c2r_pair<NX>(g1, g3, ny, j, t, xl, tw_n, v);   // uy + i d omega/dy
+    x_c2r_pair<NX, EM, SPLIT>(g1, g3, ny, j, t, xl, tw_n, v);   // uy + i d omega/dy
 #pragma unroll
     for (int e = 0; e < E; ++e) nl[e] = (nl[e] + v[e].x * v[e].y) * scale;
     // forward x transform of the real product row: pack z[m] = nl[2m] + i nl[2m+1]
@@ -193,21 +217,22 @@ __global__ __launch_bounds__(XShape<NX>::NT) void k_csh_x(const double2* g0, con
     // code on the unused upper half of the row buffer so every barrier matches).
     double* xr = reinterpret_cast<double*>(xl);
 #pragma unroll
-    for (int e = 0; e < E; ++e) xr[rf_final_idx<NX>(t, e)] = nl[e];
+    for (int e = 0; e < E; ++e) xr[rf_final_idx<NX, EM>(t, e)] = nl[e];
     __syncthreads();
-    double2 z[RF<M>::E];
+    double2 z[EH];
     const int tm = t % S::TM;
 #pragma unroll
-    for (int e = 0; e < RF<M>::E; ++e) {
-        const int m = rf_first_idx<M>(tm, e);
+    for (int e = 0; e < EH; ++e) {
+        const int m = rf_first_idx<M, EM>(tm, e);
         z[e] = make_double2(xr[2 * m], xr[2 * m + 1]);
     }
     __syncthreads();
-    double2* zl = xl + (t < S::TM ? 0 : RF<M>::LDS);
-    rf_fft<M, -1>(z, tm, zl, tw_m);
+    constexpr int HALF = SPLIT ? RF<M, EM>::LDS / 2 : RF<M, EM>::LDS;  // upper half, complex elements
+    double2* zl = xl + (t < S::TM ? 0 : HALF);
+    rf_fft<M, -1, EM, false, SPLIT>(z, tm, zl, tw_m);
     if (t < S::TM) {
 #pragma unroll
-        for (int e = 0; e < RF<M>::E; ++e) xl[rf_pad(rf_final_idx<M>(tm, e))] = z[e];
+        for (int e = 0; e < EH; ++e) xl[rf_pad(rf_final_idx<M, EM>(tm, e))] = z[e];
     }
     __syncthreads();
     for (int k = t; k < M; k += T) {  // consecutive t -> consecutive kx
@@ -310,21 +335,70 @@ static int launch_yinv(const CshPlan& p, const double2* w, const CshPhys& ph, hi
 {
     using S = YShape<NY>;
     hipLaunchKernelGGL(k_csh_yinv<NY>, dim3((p.nh + S::CB - 1) / S::CB), dim3(S::NT), 0, st, w, p.nh,
-                       p.kx.as<double>(), p.ky.as<double>(), p.g[0].as<double2>(), p.g[1].as<double2>(),
-                       p.g[2].as<double2>(), p.g[3].as<double2>(), p.tw_y, ph.lambda, ph.screened);
+                       p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(),
+                       p.tw_y, ph.lambda, ph.screened);
     return TARA_OK;
+}
+
+// x-pass shapes the autotuner chooses among: {EM, RB, SPLIT}.  Shape 0 is
+// RB = min(8, 1024 / T) at EM = 8 without splitting (the largest workgroup).
+struct XCand {
+    int em, rb, split;
+};
+constexpr XCand kXCands[] = {{8, 0, 0}, {16, 2, 1}, {8, 1, 1}, {16, 4, 1}, {8, 2, 1}, {16, 1, 0}};
+constexpr int kNumXCands = sizeof(kXCands) / sizeof(kXCands[0]);
+
+template <int NX, int EM>
+constexpr int default_rb()
+{
+    return (1024 / RF<NX, EM>::T) < 1 ? 1 : (1024 / RF<NX, EM>::T) > 8 ? 8 : (1024 / RF<NX, EM>::T);
+}
+
+template <int NX, int EM, int RB, bool SPLIT>
+static int launch_x_shape(const CshPlan& p, hipStream_t st)
+{
+    using S = XShape<NX, EM, RB, SPLIT>;
+    if constexpr (!S::OK) {
+        set_error("x pass shape unsupported at Nx=%d", NX);
+        return TARA_ERR_UNSUPPORTED;
+    } else {
+        TARA_CHECK(p.ny % RB == 0, TARA_ERR_UNSUPPORTED, "Ny=%d not a multiple of %d", p.ny, RB);
+        const double scale = 1.0 / ((double)p.nx * (double)p.ny * (double)p.nx * (double)p.ny);
+        const double2* twn = EM == 16 ? p.tw_xn16 : p.tw_xn;
+        const double2* twm = EM == 16 ? p.tw_x16 : p.tw_x;
+        hipLaunchKernelGGL((k_csh_x<NX, EM, RB, SPLIT>), dim3(p.ny / RB), dim3(S::NT), 0, st, p.g[0].as<double2>(),
+                           p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(), p.ny, scale,
+                           p.h.as<double2>(), twn, twm, p.rctw);
+        return TARA_OK;
+    }
 }
 
 template <int NX>
 static int launch_x(const CshPlan& p, hipStream_t st)
 {
-    using S = XShape<NX>;
-    TARA_CHECK(p.ny % S::RB == 0, TARA_ERR_UNSUPPORTED, "Ny=%d not a multiple of %d", p.ny, S::RB);
-    const double scale = 1.0 / ((double)p.nx * (double)p.ny * (double)p.nx * (double)p.ny);
-    hipLaunchKernelGGL(k_csh_x<NX>, dim3(p.ny / S::RB), dim3(S::NT), 0, st, p.g[0].as<double2>(),
-                       p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(), p.ny, scale,
-                       p.h.as<double2>(), p.tw_xn, p.tw_x, p.rctw);
-    return TARA_OK;
+    switch (p.xcfg) {
+        case 0: return launch_x_shape<NX, 8, default_rb<NX, 8>(), false>(p, st);
+        case 1: return launch_x_shape<NX, 16, 2, true>(p, st);
+        case 2: return launch_x_shape<NX, 8, 1, true>(p, st);
+        case 3: return launch_x_shape<NX, 16, 4, true>(p, st);
+        case 4: return launch_x_shape<NX, 8, 2, true>(p, st);
+        case 5: return launch_x_shape<NX, 16, 1, false>(p, st);
+        default: set_error("x pass shape %d unknown", p.xcfg); return TARA_ERR_ARG;
+    }
+}
+
+template <int NX>
+static bool x_shape_ok(int c, int ny)
+{
+    switch (c) {
+        case 0: return XShape<NX, 8, default_rb<NX, 8>(), false>::OK && ny % default_rb<NX, 8>() == 0;
+        case 1: return XShape<NX, 16, 2, true>::OK && ny % 2 == 0;
+        case 2: return XShape<NX, 8, 1, true>::OK;
+        case 3: return XShape<NX, 16, 4, true>::OK && ny % 4 == 0;
+        case 4: return XShape<NX, 8, 2, true>::OK && ny % 2 == 0;
+        case 5: return XShape<NX, 16, 1, false>::OK;
+        default: return false;
+    }
 }
 
 template <int NY>
@@ -363,6 +437,12 @@ static int yinv(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream
     TARA_CSH_SWITCH(p.ny, launch_yinv, p, w, ph, st)
 }
 static int xpass(const CshPlan& p, hipStream_t st) { TARA_CSH_SWITCH(p.nx, launch_x, p, st) }
+template <int NX>
+static int shape_ok_n(int c, int ny)
+{
+    return x_shape_ok<NX>(c, ny) ? 1 : 0;
+}
+static int shape_ok(const CshPlan& p, int c) { TARA_CSH_SWITCH(p.nx, shape_ok_n, c, p.ny) }
 static int yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 {
     TARA_CSH_SWITCH(p.ny, launch_yfwd, p, mode, a, st)
@@ -379,8 +459,10 @@ int csh_plan_init(CshPlan& p, int nx, int ny)
     p.nh = nx / 2 + 1;
     const size_t ns = (size_t)p.nh * ny;
     int rc;
-    for (int f = 0; f < 4; ++f)
+    for (int f = 0; f < 4; ++f) {
         if ((rc = p.g[f].alloc(ns * sizeof(double2)))) return rc;
+        TARA_HIP(hipMemset(p.g[f].p, 0, ns * sizeof(double2)));
+    }
     if ((rc = p.h.alloc(ns * sizeof(double2)))) return rc;
     // wavenumber and dealias tables, with derive2csh's own expressions
     std::vector<double> kx(p.nh), ky(ny);
@@ -412,10 +494,27 @@ int csh_plan_init(CshPlan& p, int nx, int ny)
     p.tw_y = twiddles(ny);
     p.tw_x = twiddles(nx / 2);
     p.tw_xn = twiddles(nx);
+    p.tw_x16 = twiddles(nx / 2, 16);
+    p.tw_xn16 = twiddles(nx, 16);
     p.rctw = rc_twiddles(nx);
-    TARA_CHECK(p.tw_y && p.tw_x && p.tw_xn && p.rctw, TARA_ERR_ALLOC, "twiddle table allocation failed");
+    TARA_CHECK(p.tw_y && p.tw_x && p.tw_xn && p.tw_x16 && p.tw_xn16 && p.rctw, TARA_ERR_ALLOC,
+               "twiddle table allocation failed");
+    p.xcfg = csh_default_xshape(nx);
+    if (shape_ok(p, p.xcfg) != 1) p.xcfg = 0;
     return TARA_OK;
 }
+
+int csh_default_xshape(int nx)
+{
+    (void)nx;
+    return 0;
+}
+
+int csh_num_xshapes() { return kNumXCands; }
+
+int csh_xshape_ok(const CshPlan& p, int c) { return c >= 0 && c < kNumXCands && shape_ok(p, c) == 1; }
+
+int csh_xpass(const CshPlan& p, hipStream_t st) { return xpass(p, st); }
 
 int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& ph, double dt, double2* base,
                   double2* aux, double2* next, hipStream_t st, EventTimer* tm)

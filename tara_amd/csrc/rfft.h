@@ -19,7 +19,8 @@
 // of 16-byte accesses per cycle (128 B), and with this padding the radix-8
 // scatter (stride 8 elements) and every gather of the schedule put 8
 // consecutive lanes on 8 distinct 16-byte bank groups.
-// Radix schedule and twiddle tables are those of fft_core.h for (N, EM).
+// Radix schedule and twiddle tables are those of fft_core.h for (N, EM);
+// EM = 16 runs radix-16 passes (4096 = 16^3: two exchanges instead of three).
 #pragma once
 #include "fft_core.h"
 
@@ -108,32 +109,52 @@ __device__ __forceinline__ void rf_compute(double2 (&v)[(RF<N, EM>::E)], int t, 
     }
 }
 
-template <int N, int L, int DIR, int EM, bool WAVE>
+template <int N, int L, int DIR, int EM, bool WAVE, bool SPLIT>
 __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw)
 {
     constexpr int E = RF<N, EM>::E;
     constexpr int R = fft_radix(N / L, E);
     rf_compute<N, L, DIR, EM>(v, t, tw);
     if constexpr (L * R < N) {
+        if constexpr (SPLIT) {
+            // real and imaginary parts in two rounds through a buffer of
+            // doubles: half the LDS of the double2 exchange
+            double* d = reinterpret_cast<double*>(lds);
 #pragma unroll
-        for (int e = 0; e < E; ++e) lds[rf_pad(rf_out_idx<N, L, EM>(t, e))] = v[e];
-        rf_sync<WAVE>();
+            for (int e = 0; e < E; ++e) d[rf_pad(rf_out_idx<N, L, EM>(t, e))] = v[e].x;
+            rf_sync<WAVE>();
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = lds[rf_pad(rf_in_idx<N, L * R, EM>(t, e))];
-        rf_sync<WAVE>();
-        rf_rec<N, L * R, DIR, EM, WAVE>(v, t, lds, tw);
+            for (int e = 0; e < E; ++e) v[e].x = d[rf_pad(rf_in_idx<N, L * R, EM>(t, e))];
+            rf_sync<WAVE>();
+#pragma unroll
+            for (int e = 0; e < E; ++e) d[rf_pad(rf_out_idx<N, L, EM>(t, e))] = v[e].y;
+            rf_sync<WAVE>();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e].y = d[rf_pad(rf_in_idx<N, L * R, EM>(t, e))];
+            rf_sync<WAVE>();
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) lds[rf_pad(rf_out_idx<N, L, EM>(t, e))] = v[e];
+            rf_sync<WAVE>();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = lds[rf_pad(rf_in_idx<N, L * R, EM>(t, e))];
+            rf_sync<WAVE>();
+        }
+        rf_rec<N, L * R, DIR, EM, WAVE, SPLIT>(v, t, lds, tw);
     }
 }
 
 // Transform one sequence.  v: inputs at rf_first_idx positions; on return the
-// outputs at rf_final_idx positions.  lds: this sequence's RF<N, EM>::LDS-element
-// exchange buffer.  tw: the twiddle table of the (N, RF<N, EM>::E) schedule.
-// With WAVE = false it contains workgroup barriers: every thread of the
-// workgroup must call it the same number of times.
-template <int N, int DIR, int EM = 8, bool WAVE = false>
+// outputs at rf_final_idx positions.  lds: this sequence's exchange buffer of
+// RF<N, EM>::LDS complex elements (SPLIT = false) or RF<N, EM>::LDS doubles
+// (SPLIT = true: real and imaginary parts exchanged in two rounds).  tw: the
+// twiddle table of the (N, RF<N, EM>::E) schedule.  With WAVE = false it
+// contains workgroup barriers: every thread of the workgroup must call it the
+// same number of times.
+template <int N, int DIR, int EM = 8, bool WAVE = false, bool SPLIT = false>
 __device__ __forceinline__ void rf_fft(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw)
 {
-    rf_rec<N, 1, DIR, EM, WAVE>(v, t, lds, tw);
+    rf_rec<N, 1, DIR, EM, WAVE, SPLIT>(v, t, lds, tw);
 }
 
 // Reorders registers from final (output) positions to first-pass (input)

@@ -176,3 +176,37 @@ def test_rk4_4096_one_step_consistency():
     k4 = d(w + dt * k3)
     ref = w + (dt / 6.0) * (k1 + 2 * k2 + 2 * k3 + k4)
     assert rel_l2(got, ref) < 1e-14
+
+
+# ------------------------------------------------------------------ x-pass shapes
+@pytest.mark.parametrize("nx,ny", [(64, 64), (256, 512), (1024, 1024), (4096, 256)])
+def test_every_xshape_matches_oracle(nx, ny):
+    """Every x-pass shape the autotuner may choose gives the same AB2 step
+    (w + 3/2 dt f from zero history) as the oracle."""
+    print(f"xshapes {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(nx + ny)
+    w = O.r2c(np.asfortranarray(rng.standard_normal((nx, ny))))
+    dt, nu = 1e-3, 1e-3
+    want = w + 1.5 * dt * O.derive2csh(w, nx, nu)
+    ran = 0
+    for shape in range(T.lib().tara_csh_num_xshapes()):
+        s = T.CshSolver(nx, ny, nu, dt, T.SCHEME_AB2)
+        try:
+            s.set_xshape(shape)
+        except T.TaraError:
+            s.close()
+            continue
+        s.upload(w)
+        s.advance(1)
+        got = s.download()
+        s.close()
+        ran += 1
+        assert rel_l2(got - w, want - w) < 1e-13, shape
+    assert ran >= (2 if nx >= 256 else 1)
+
+
+def test_tune_picks_a_running_shape():
+    s = T.CshSolver(512, 512, 1e-3, 1e-3, T.SCHEME_RK4)
+    chosen, ms = s.tune(reps=2)
+    assert ms[chosen] > 0 and ms[chosen] == min(m for m in ms if m > 0)
+    s.close()
