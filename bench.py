@@ -98,6 +98,58 @@ def bench_mhd(steps, warmup, n=256):
                                    "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}}
 
 
+def bench_ensemble(steps, warmup, nruns=512, seed=7):
+    """Secondary line: batched ensemble of 512 independent 2D runs, grids drawn
+    uniformly from {128, 256, 512}^2, nu log-uniform in [1e-4, 1e-2], k0 uniform
+    in [3, 8], dt = 5e-4, RK4, one hipGraph replay per step; against the same
+    512 runs done one after another on the single-run solver."""
+    import numpy as np
+
+    import tara_amd as T
+    from tara_amd.ic import random_phase_vorticity
+
+    rng = np.random.default_rng(seed)
+    sizes = [int(v) for v in rng.choice([128, 256, 512], size=nruns)]
+    nus = 10.0 ** rng.uniform(-4, -2, size=nruns)
+    k0s = rng.uniform(3, 8, size=nruns)
+    e = T.Ensemble(sizes, nus, 5e-4, T.SCHEME_RK4)
+    for r in range(nruns):
+        e.upload(r, random_phase_vorticity(sizes[r], seed=1000 + r, k0=k0s[r]))
+    e.tune(3)
+    e.advance(warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.advance(steps)
+    e.sync()
+    t_ens = (time.perf_counter() - t0) / steps
+    e.close()
+    seq = 0.0
+    singles = {}
+    for n in (128, 256, 512):
+        s = T.CshSolver(n, n, 1e-3, 5e-4, T.SCHEME_RK4)
+        s.upload(random_phase_vorticity(n, seed=5))
+        s.tune(reps=3)
+        s.advance(warmup)
+        s.sync()
+        reps = 4 * steps
+        t0 = time.perf_counter()
+        s.advance(reps)
+        s.sync()
+        singles[n] = (time.perf_counter() - t0) / reps
+        s.close()
+        seq += sizes.count(n) * singles[n]
+    pts = float(sum(n * n for n in sizes))
+    S_all = sum(16.0 * (n // 2 + 1) * n for n in sizes)
+    return {"workload": f"ensemble of {nruns} independent 2D runs, grids U{{128,256,512}}^2, nu log-U[1e-4,1e-2], "
+                        "RK4, dt 5e-4, FP64, hipGraph replay",
+            "value": pts / t_ens, "unit": "grid-pt steps/s", "ms_per_step": t_ens * 1e3,
+            "runs_per_size": {n: sizes.count(n) for n in (128, 256, 512)},
+            "sequential_value": pts / seq, "sequential_ms_per_step": seq * 1e3,
+            "single_run_ms_per_step": {n: round(v * 1e3, 4) for n, v in singles.items()},
+            "speedup_vs_sequential": seq / t_ens,
+            "step_model_bytes": 60 * S_all, "step_model_GBps": round(60 * S_all / t_ens / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +158,7 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-3d", action="store_true", help="skip the secondary 3D MHD measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the secondary ensemble measurement")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -155,6 +208,7 @@ def main():
     kernels = {names[c]: {"avg_ms": round(avg[c], 4), "launches": counts[c],
                           "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}
     mhd = None if a.no_3d else bench_mhd(a.steps, a.warmup)
+    ens = None if a.no_ensemble else bench_ensemble(min(a.steps, 20), a.warmup)
     out = {
         "metric": METRIC,
         "value": n * n / (ms_per_step * 1e-3),
@@ -182,6 +236,7 @@ def main():
         "state_finite": finite,
         "cpu_baseline": cpu,
         "secondary_3d_mhd": mhd,
+        "secondary_ensemble": ens,
     }
     print(json.dumps(out))
 

@@ -87,6 +87,26 @@ int tara_csh_num_xshapes(void);
 int tara_csh_tune(tara_csh* ctx, int reps, int* chosen, double* ms);
 int tara_csh_set_xshape(tara_csh* ctx, int shape);
 
+/* ------------------------------------------------ batched ensemble (csh)
+ * No counterpart in the reference (one run per process there): nruns
+ * independent 2D incompressible runs (derive2csh physics, grid n[r] x n[r],
+ * viscosity nu[r], common dt, classic RK4 or AB2).  Runs of one grid size form a
+ * bucket whose passes are single batched launches; one step of every bucket is
+ * captured once in a hipGraph (tara_ens_use_graph(ctx, 0) launches eagerly
+ * instead) and replayed per step.  upload/download take one run's (Nh, N) half
+ * spectrum in the reference layout; upload zeroes that run's AB2 history.
+ * tara_ens_tune runs the x-pass autotuner per bucket.
+ */
+typedef struct tara_ens tara_ens;
+int tara_ens_create(int nruns, const int* n, const double* nu, double dt, int scheme, tara_ens** out);
+int tara_ens_upload(tara_ens* ctx, int run, const double* omegak);
+int tara_ens_download(tara_ens* ctx, int run, double* omegak);
+int tara_ens_advance(tara_ens* ctx, int nsteps);
+int tara_ens_sync(tara_ens* ctx);
+int tara_ens_use_graph(tara_ens* ctx, int on);
+int tara_ens_tune(tara_ens* ctx, int reps);
+int tara_ens_destroy(tara_ens* ctx);
+
 /* ------------------------------------- 2D screened vorticity (css family)
  * Replaces src/derive2css.f95:6 derive2css(Nx,Ny,Nh,alpha,nu,time,omegak,
  * omegak_new,dt_omegak_old,dt_omegak_new): psi = omega/(k^2 + lambda) with

@@ -83,6 +83,14 @@ def lib():
         L.tara_csh_num_xshapes.argtypes = []
         L.tara_csh_tune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]
         L.tara_csh_set_xshape.argtypes = [C.c_void_p, C.c_int]
+        L.tara_ens_create.argtypes = [C.c_int, C.POINTER(C.c_int), _dp, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
+        L.tara_ens_upload.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.tara_ens_download.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.tara_ens_advance.argtypes = [C.c_void_p, C.c_int]
+        for n in ("tara_ens_sync", "tara_ens_destroy"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.tara_ens_use_graph.argtypes = [C.c_void_p, C.c_int]
+        L.tara_ens_tune.argtypes = [C.c_void_p, C.c_int]
         P3, P6 = C.c_void_p * 3, C.c_void_p * 6
         L.tara_derive2coh.argtypes = [C.c_int] * 3 + [C.POINTER(CohParams), P3, P3]
         L.tara_ab2coh.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, P3, P3]
@@ -262,6 +270,56 @@ class CshSolver:
     def close(self):
         if self._h:
             lib().tara_csh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Batched ensemble of independent 2D incompressible runs (derive2csh
+    physics): run r has an n[r] x n[r] grid and viscosity nu[r]; all share dt and
+    the scheme.  Buckets of one grid size advance in batched launches; a step is
+    replayed from a captured hipGraph."""
+
+    def __init__(self, n, nu, dt, scheme=SCHEME_RK4):
+        self.n = [int(v) for v in n]
+        arr_n = (C.c_int * len(self.n))(*self.n)
+        arr_nu = (C.c_double * len(self.n))(*[float(v) for v in nu])
+        h = C.c_void_p()
+        _check(lib().tara_ens_create(len(self.n), arr_n, arr_nu, dt, scheme, C.byref(h)))
+        self._h = h
+
+    def upload(self, run, omegak):
+        n = self.n[run]
+        w = _spec(omegak)
+        assert w.shape == (n // 2 + 1, n)
+        _check(lib().tara_ens_upload(self._h, run, _p(w)))
+
+    def download(self, run):
+        n = self.n[run]
+        out = np.zeros((n // 2 + 1, n), np.complex128, order="F")
+        _check(lib().tara_ens_download(self._h, run, _p(out)))
+        return out
+
+    def advance(self, nsteps):
+        _check(lib().tara_ens_advance(self._h, nsteps))
+
+    def sync(self):
+        _check(lib().tara_ens_sync(self._h))
+
+    def use_graph(self, on):
+        _check(lib().tara_ens_use_graph(self._h, 1 if on else 0))
+
+    def tune(self, reps=3):
+        _check(lib().tara_ens_tune(self._h, reps))
+
+    def close(self):
+        if self._h:
+            lib().tara_ens_destroy(self._h)
             self._h = None
 
     def __del__(self):

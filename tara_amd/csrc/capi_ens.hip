@@ -1,0 +1,268 @@
+// C ABI of the batched ensemble: many independent 2D incompressible runs
+// (derive2csh physics) of possibly different grid sizes and viscosities,
+// advanced together.  Runs are grouped by grid size into buckets; each bucket's
+// passes are single batched launches over all its runs (blockIdx.y = run), the
+// buckets run on streams of their own, and one whole step (every bucket, every
+// stage) is captured once in a hipGraph and replayed per step.
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "csh2d.h"
+#include "layout.h"
+
+using namespace tara;
+
+namespace {
+
+struct Bucket {
+    int n = 0;
+    std::vector<int> runs;  // run ids, in bucket order
+    CshPlan plan;           // batch = runs.size()
+    DevBuf w, aux, next, stage, nus;
+    CshPhys ph;
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+};
+
+}  // namespace
+
+struct tara_ens {
+    std::vector<std::unique_ptr<Bucket>> b;
+    std::vector<std::pair<int, int>> where;  // run -> (bucket, slot)
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr;
+    double dt = 0;
+    int scheme = 0;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void drop_graph(tara_ens* e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+// One step of every bucket, forked onto the bucket streams and joined back.
+int enqueue_step(tara_ens* e)
+{
+    TARA_HIP(hipEventRecord(e->fork, e->st));
+    for (auto& bp : e->b) {
+        Bucket& b = *bp;
+        TARA_HIP(hipStreamWaitEvent(b.st, e->fork, 0));
+        double2* w = b.w.as<double2>();
+        double2* aux = b.aux.as<double2>();
+        double2* nx = b.next.as<double2>();
+        int rc;
+        if (e->scheme == TARA_SCHEME_AB2) {
+            if ((rc = csh_rhs_stage(b.plan, w, CSH_AB2, b.ph, e->dt, w, aux, nullptr, b.st, nullptr))) return rc;
+        } else {
+            if ((rc = csh_rhs_stage(b.plan, w, CSH_RK4_1, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
+            if ((rc = csh_rhs_stage(b.plan, nx, CSH_RK4_2, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
+            if ((rc = csh_rhs_stage(b.plan, nx, CSH_RK4_3, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
+            if ((rc = csh_rhs_stage(b.plan, nx, CSH_RK4_4, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
+        }
+        TARA_HIP(hipEventRecord(b.done, b.st));
+        TARA_HIP(hipStreamWaitEvent(e->st, b.done, 0));
+    }
+    return TARA_OK;
+}
+
+int capture(tara_ens* e)
+{
+    drop_graph(e);
+    TARA_HIP(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(e);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    TARA_HIP(ce);
+    e->graph = g;
+    TARA_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return TARA_OK;
+}
+
+int run_slot(tara_ens* e, int run, Bucket** b, int* slot)
+{
+    TARA_CHECK(e && run >= 0 && run < (int)e->where.size(), TARA_ERR_ARG, "run %d out of range", run);
+    *b = e->b[e->where[run].first].get();
+    *slot = e->where[run].second;
+    return TARA_OK;
+}
+
+}  // namespace
+
+extern "C" int tara_ens_create(int nruns, const int* n, const double* nu, double dt, int scheme, tara_ens** out)
+{
+    TARA_CHECK(out && n && nu && nruns > 0, TARA_ERR_ARG, "bad argument");
+    TARA_CHECK(scheme == TARA_SCHEME_AB2 || scheme == TARA_SCHEME_RK4, TARA_ERR_ARG, "unknown scheme %d", scheme);
+    *out = nullptr;
+    auto e = std::make_unique<tara_ens>();
+    e->dt = dt;
+    e->scheme = scheme;
+    std::map<int, int> index;  // grid -> bucket
+    e->where.resize(nruns);
+    for (int r = 0; r < nruns; ++r) {
+        auto it = index.find(n[r]);
+        if (it == index.end()) {
+            it = index.emplace(n[r], (int)e->b.size()).first;
+            e->b.push_back(std::make_unique<Bucket>());
+            e->b.back()->n = n[r];
+        }
+        Bucket& b = *e->b[it->second];
+        e->where[r] = {it->second, (int)b.runs.size()};
+        b.runs.push_back(r);
+    }
+    int rc;
+    for (auto& bp : e->b) {
+        Bucket& b = *bp;
+        const int cnt = (int)b.runs.size();
+        if ((rc = csh_plan_init(b.plan, b.n, b.n, cnt))) return rc;
+        const size_t bytes = b.plan.bstride * sizeof(double2);
+        if ((rc = b.w.alloc(cnt * bytes)) || (rc = b.aux.alloc(cnt * bytes)) || (rc = b.stage.alloc(bytes)) ||
+            (rc = b.nus.alloc(cnt * sizeof(double))))
+            return rc;
+        if (scheme == TARA_SCHEME_RK4 && (rc = b.next.alloc(cnt * bytes))) return rc;
+        TARA_HIP(hipMemset(b.w.p, 0, b.w.bytes));
+        TARA_HIP(hipMemset(b.aux.p, 0, b.aux.bytes));
+        if (b.next.p) TARA_HIP(hipMemset(b.next.p, 0, b.next.bytes));
+        std::vector<double> v(cnt);
+        for (int k = 0; k < cnt; ++k) v[k] = nu[b.runs[k]];
+        TARA_HIP(hipMemcpy(b.nus.p, v.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+        b.ph.nu = v[0];
+        b.ph.nus = b.nus.as<double>();
+        TARA_HIP(hipStreamCreate(&b.st));
+        TARA_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    }
+    TARA_HIP(hipStreamCreate(&e->st));
+    TARA_HIP(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+    *out = e.release();
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_upload(tara_ens* e, int run, const double* omegak)
+{
+    TARA_CHECK(omegak, TARA_ERR_ARG, "null array");
+    Bucket* b = nullptr;
+    int slot = 0, rc;
+    if ((rc = run_slot(e, run, &b, &slot))) return rc;
+    const CshPlan& p = b->plan;
+    TARA_HIP(hipStreamSynchronize(e->st));
+    TARA_HIP(hipMemcpy(b->stage.p, omegak, b->stage.bytes, hipMemcpyHostToDevice));
+    if ((rc = to_kymajor(b->stage.as<double2>(), b->w.as<double2>() + slot * p.bstride, p.nh, p.ny, 1, nullptr)))
+        return rc;
+    TARA_HIP(hipMemset(b->aux.as<double2>() + slot * p.bstride, 0, p.bstride * sizeof(double2)));
+    TARA_HIP(hipDeviceSynchronize());
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_download(tara_ens* e, int run, double* omegak)
+{
+    TARA_CHECK(omegak, TARA_ERR_ARG, "null array");
+    Bucket* b = nullptr;
+    int slot = 0, rc;
+    if ((rc = run_slot(e, run, &b, &slot))) return rc;
+    const CshPlan& p = b->plan;
+    TARA_HIP(hipStreamSynchronize(e->st));
+    if ((rc = from_kymajor(b->w.as<double2>() + slot * p.bstride, b->stage.as<double2>(), p.nh, p.ny, 1, nullptr)))
+        return rc;
+    TARA_HIP(hipMemcpy(omegak, b->stage.p, b->stage.bytes, hipMemcpyDeviceToHost));
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_advance(tara_ens* e, int nsteps)
+{
+    TARA_CHECK(e && nsteps >= 0, TARA_ERR_ARG, "bad argument");
+    int rc;
+    if (e->use_graph && !e->exec && (rc = capture(e))) return rc;
+    for (int s = 0; s < nsteps; ++s) {
+        if (e->use_graph) {
+            TARA_HIP(hipGraphLaunch(e->exec, e->st));
+        } else if ((rc = enqueue_step(e))) {
+            return rc;
+        }
+    }
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_sync(tara_ens* e)
+{
+    TARA_CHECK(e, TARA_ERR_ARG, "null context");
+    TARA_HIP(hipStreamSynchronize(e->st));
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_use_graph(tara_ens* e, int on)
+{
+    TARA_CHECK(e, TARA_ERR_ARG, "null context");
+    TARA_HIP(hipStreamSynchronize(e->st));
+    e->use_graph = on != 0;
+    drop_graph(e);
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_tune(tara_ens* e, int reps)
+{
+    TARA_CHECK(e && reps > 0, TARA_ERR_ARG, "bad argument");
+    TARA_HIP(hipStreamSynchronize(e->st));
+    hipEvent_t e0, e1;
+    TARA_HIP(hipEventCreate(&e0));
+    TARA_HIP(hipEventCreate(&e1));
+    int rc = TARA_OK;
+    for (auto& bp : e->b) {
+        Bucket& b = *bp;
+        int best = b.plan.xcfg;
+        float best_ms = -1.f;
+        for (int k = 0; k < csh_num_xshapes() && rc == TARA_OK; ++k) {
+            if (!csh_xshape_ok(b.plan, k)) continue;
+            b.plan.xcfg = k;
+            if ((rc = csh_xpass(b.plan, b.st))) break;
+            (void)hipEventRecord(e0, b.st);
+            for (int r = 0; r < reps && rc == TARA_OK; ++r) rc = csh_xpass(b.plan, b.st);
+            (void)hipEventRecord(e1, b.st);
+            float t = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+                rc = TARA_ERR_HIP;
+                break;
+            }
+            if (best_ms < 0 || t < best_ms) {
+                best_ms = t;
+                best = k;
+            }
+        }
+        b.plan.xcfg = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    drop_graph(e);  // the captured launches carry the old shapes
+    return rc;
+}
+
+extern "C" int tara_ens_destroy(tara_ens* e)
+{
+    if (!e) return TARA_OK;
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    drop_graph(e);
+    for (auto& bp : e->b) {
+        if (bp->st) {
+            (void)hipStreamSynchronize(bp->st);
+            (void)hipStreamDestroy(bp->st);
+        }
+        if (bp->done) (void)hipEventDestroy(bp->done);
+    }
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+    return TARA_OK;
+}

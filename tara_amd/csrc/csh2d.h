@@ -31,6 +31,8 @@ struct EventTimer {
 // G0..G3 (after the inverse y pass) and H (after x), all ky-major (nh x ny).
 struct CshPlan {
     int nx = 0, ny = 0, nh = 0;
+    int batch = 1;                   // independent runs of this grid (blockIdx.y), fields packed
+    size_t bstride = 0;              // per run: nh * ny complex elements
     DevBuf g[4], h;
     DevBuf kx, ky, imin;
     const double2* tw_y = nullptr;
@@ -51,9 +53,12 @@ struct CshPhys {
     double alpha = 1.0;
     double lambda = 0.0;
     int screened = 0;
+    const double* nus = nullptr;  // batched runs: device array of nu per run (overrides nu)
 };
 
-int csh_plan_init(CshPlan& p, int nx, int ny);
+// batch: number of independent runs of this grid the launches cover; every
+// per-run field (state, G0..G3, H) is packed run after run, bstride apart.
+int csh_plan_init(CshPlan& p, int nx, int ny, int batch = 1);
 // One right-hand-side evaluation on stage input ws followed by the fused
 // update selected by mode (see k_csh_yfwd).  Timing classes: 0 inverse y,
 // 1 x pass, 2 forward y + update.

@@ -131,8 +131,15 @@ template <int NY>
 __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, int nh,
                                                               double2* __restrict__ g0, double2* __restrict__ g1,
                                                               double2* __restrict__ g2, double2* __restrict__ g3,
-                                                              const double2* tw, double lam, int screened)
+                                                              const double2* tw, double lam, int screened, size_t bs)
 {
+    // batched runs: blockIdx.y selects the run, every field is strided by bs
+    const size_t rb = blockIdx.y * bs;
+    w += rb;
+    g0 += rb;
+    g1 += rb;
+    g2 += rb;
+    g3 += rb;
     using S = YShape<NY>;
     constexpr int E = RF<NY>::E, T = S::T;
     __shared__ double2 lds[S::CB * RF<NY>::LDS];
@@ -195,8 +202,14 @@ struct XShape {
 template <int NX, int EM, int RB, bool SPLIT>
 __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
     const double2* g0, const double2* g1, const double2* g2, const double2* g3, int ny, double scale,
-    double2* __restrict__ h, const double2* tw_n, const double2* tw_m, const double2* rctw)
+    double2* __restrict__ h, const double2* tw_n, const double2* tw_m, const double2* rctw, size_t bs)
 {
+    const size_t rb = blockIdx.y * bs;  // batched runs
+    g0 += rb;
+    g1 += rb;
+    g2 += rb;
+    g3 += rb;
+    h += rb;
     using S = XShape<NX, EM, RB, SPLIT>;
     static_assert(S::OK, "x pass shape does not fit");
     constexpr int E = RF<NX, EM>::E, EH = RF<S::M, EM>::E, T = S::T, M = S::M;
@@ -258,11 +271,20 @@ struct YfwdArgs {
     double nu, dt;
     double alpha, lam;     // screened form: alpha * NL, viscous term on k^2 ws/(k^2 + lam)
     int screened;
+    size_t bs;             // batched runs: field stride per run (blockIdx.y)
+    const double* nus;     // batched runs: viscosity per run (null: nu)
 };
 
 template <int NY, int MODE>
 __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const double2* tw)
 {
+    const size_t rb = blockIdx.y * a.bs;
+    a.h += rb;
+    a.ws += rb;
+    if (a.base) a.base += rb;
+    if (a.aux) a.aux += rb;
+    if (a.next) a.next += rb;
+    if (a.nus) a.nu = a.nus[blockIdx.y];
     using S = YShape<NY>;
     constexpr int E = RF<NY>::E, T = S::T;
     __shared__ double2 lds[S::CB * RF<NY>::LDS];
@@ -334,9 +356,9 @@ template <int NY>
 static int launch_yinv(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream_t st)
 {
     using S = YShape<NY>;
-    hipLaunchKernelGGL(k_csh_yinv<NY>, dim3((p.nh + S::CB - 1) / S::CB), dim3(S::NT), 0, st, w, p.nh,
+    hipLaunchKernelGGL(k_csh_yinv<NY>, dim3((p.nh + S::CB - 1) / S::CB, p.batch), dim3(S::NT), 0, st, w, p.nh,
                        p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(),
-                       p.tw_y, ph.lambda, ph.screened);
+                       p.tw_y, ph.lambda, ph.screened, p.bstride);
     return TARA_OK;
 }
 
@@ -366,9 +388,9 @@ static int launch_x_shape(const CshPlan& p, hipStream_t st)
         const double scale = 1.0 / ((double)p.nx * (double)p.ny * (double)p.nx * (double)p.ny);
         const double2* twn = EM == 16 ? p.tw_xn16 : p.tw_xn;
         const double2* twm = EM == 16 ? p.tw_x16 : p.tw_x;
-        hipLaunchKernelGGL((k_csh_x<NX, EM, RB, SPLIT>), dim3(p.ny / RB), dim3(S::NT), 0, st, p.g[0].as<double2>(),
-                           p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(), p.ny, scale,
-                           p.h.as<double2>(), twn, twm, p.rctw);
+        hipLaunchKernelGGL((k_csh_x<NX, EM, RB, SPLIT>), dim3(p.ny / RB, p.batch), dim3(S::NT), 0, st,
+                           p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(),
+                           p.ny, scale, p.h.as<double2>(), twn, twm, p.rctw, p.bstride);
         return TARA_OK;
     }
 }
@@ -405,7 +427,7 @@ template <int NY>
 static int launch_yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 {
     using S = YShape<NY>;
-    dim3 g((p.nh + S::CB - 1) / S::CB), b(S::NT);
+    dim3 g((p.nh + S::CB - 1) / S::CB, p.batch), b(S::NT);
     switch (mode) {
         case CSH_RHS: hipLaunchKernelGGL((k_csh_yfwd<NY, CSH_RHS>), g, b, 0, st, a, p.tw_y); break;
         case CSH_AB2: hipLaunchKernelGGL((k_csh_yfwd<NY, CSH_AB2>), g, b, 0, st, a, p.tw_y); break;
@@ -449,8 +471,9 @@ static int yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 }
 
 // ------------------------------------------------------------------ host side
-int csh_plan_init(CshPlan& p, int nx, int ny)
+int csh_plan_init(CshPlan& p, int nx, int ny, int batch)
 {
+    TARA_CHECK(batch >= 1 && batch <= 65535, TARA_ERR_ARG, "batch %d out of range", batch);
     TARA_CHECK(is_pow2(nx) && is_pow2(ny) && nx >= 32 && ny >= 16 && nx <= 4096 && ny <= 4096,
                TARA_ERR_UNSUPPORTED, "derive2csh: grid %dx%d unsupported (powers of two, 32..4096 x 16..4096)",
                nx, ny);
@@ -458,12 +481,14 @@ int csh_plan_init(CshPlan& p, int nx, int ny)
     p.ny = ny;
     p.nh = nx / 2 + 1;
     const size_t ns = (size_t)p.nh * ny;
+    p.batch = batch;
+    p.bstride = ns;
     int rc;
     for (int f = 0; f < 4; ++f) {
-        if ((rc = p.g[f].alloc(ns * sizeof(double2)))) return rc;
-        TARA_HIP(hipMemset(p.g[f].p, 0, ns * sizeof(double2)));
+        if ((rc = p.g[f].alloc(batch * ns * sizeof(double2)))) return rc;
+        TARA_HIP(hipMemset(p.g[f].p, 0, batch * ns * sizeof(double2)));
     }
-    if ((rc = p.h.alloc(ns * sizeof(double2)))) return rc;
+    if ((rc = p.h.alloc(batch * ns * sizeof(double2)))) return rc;
     // wavenumber and dealias tables, with derive2csh's own expressions
     std::vector<double> kx(p.nh), ky(ny);
     std::vector<int> imin(ny);
@@ -541,6 +566,8 @@ int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& 
     a.alpha = ph.alpha;
     a.lam = ph.lambda;
     a.screened = ph.screened;
+    a.bs = p.bstride;
+    a.nus = ph.nus;
     if (tm) tm->begin(2, st);
     if ((rc = yfwd(p, mode, a, st))) return rc;
     if (tm) tm->end(2, st);

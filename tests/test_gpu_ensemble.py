@@ -1,0 +1,87 @@
+"""GPU parity of the batched ensemble (tara_ens_*): every run of a mixed-size,
+mixed-viscosity ensemble matches the same run done alone, graph replay matches
+eager launches replay after replay, and AB2 runs reproduce the reference's
+config-1 trajectory."""
+import numpy as np
+import pytest
+
+import tara_amd as T
+from conftest import rel_l2
+from tara_amd.ic import random_phase_vorticity
+
+pytestmark = pytest.mark.gpu
+
+
+def _ensemble_inputs():
+    n = [64, 128, 64, 256, 128, 64]
+    nu = [1e-2, 1e-3, 5e-3, 1e-4, 2e-3, 1e-3]
+    w0 = [random_phase_vorticity(m, seed=10 + r, k0=3.0 + r) for r, m in enumerate(n)]
+    return n, nu, w0
+
+
+def test_ensemble_matches_single_runs():
+    print("ensemble vs single runs", flush=True)
+    n, nu, w0 = _ensemble_inputs()
+    dt = 5e-4
+    e = T.Ensemble(n, nu, dt, T.SCHEME_RK4)
+    for r, w in enumerate(w0):
+        e.upload(r, w)
+    e.advance(20)
+    got = [e.download(r) for r in range(len(n))]
+    e.close()
+    for r, m in enumerate(n):
+        s = T.CshSolver(m, m, nu[r], dt, T.SCHEME_RK4)
+        s.upload(w0[r])
+        s.advance(20)
+        want = s.download()
+        s.close()
+        assert rel_l2(got[r], want) < 1e-14, r
+
+
+def test_graph_replays_match_eager():
+    print("ensemble graph vs eager", flush=True)
+    n, nu, w0 = _ensemble_inputs()
+    eg = T.Ensemble(n, nu, 5e-4, T.SCHEME_RK4)
+    ee = T.Ensemble(n, nu, 5e-4, T.SCHEME_RK4)
+    ee.use_graph(False)
+    for r, w in enumerate(w0):
+        eg.upload(r, w)
+        ee.upload(r, w)
+    for rep in range(3):
+        eg.advance(1)
+        ee.advance(1)
+        for r in range(len(n)):
+            assert np.array_equal(eg.download(r), ee.download(r)), (rep, r)
+    eg.close()
+    ee.close()
+
+
+@pytest.mark.parametrize("tag", ["asrun", "intent"])
+def test_ensemble_ab2_config1_vs_reference(golden, tag):
+    """Two copies of the config-1 run plus a differently sized run in one
+    ensemble: the 64^2 runs reproduce the reference's 2000-step trajectory."""
+    w0 = golden[f"c1_{tag}_w0"]
+    e = T.Ensemble([64, 128, 64], [0.01, 0.01, 0.01], 1e-4, T.SCHEME_AB2)
+    e.upload(0, w0)
+    e.upload(1, random_phase_vorticity(128, seed=3))
+    e.upload(2, w0)
+    e.advance(2000)
+    for r in (0, 2):
+        assert rel_l2(e.download(r), golden[f"c1_{tag}_w2000"]) < 1e-10
+    e.close()
+
+
+def test_ensemble_tune_keeps_results():
+    n, nu, w0 = _ensemble_inputs()
+    a = T.Ensemble(n, nu, 5e-4, T.SCHEME_RK4)
+    b = T.Ensemble(n, nu, 5e-4, T.SCHEME_RK4)
+    b.tune(2)
+    for r, w in enumerate(w0):
+        a.upload(r, w)
+        b.upload(r, w)
+    a.advance(5)
+    b.advance(5)
+    for r in range(len(n)):
+        assert rel_l2(b.download(r), a.download(r)) < 1e-14
+    a.close()
+    b.close()
