@@ -35,12 +35,14 @@ int tara_version(void);
 
 /* ------------------------------------------------------------------ transforms
  * FFTW 3.3.9 semantics of the calls the reference makes
- * (lib/fftw/api/f77funcs.h:206-218, 300-312: dfftw_plan_dft_{r2c,c2r}_{2d,3d}_
+ * (lib/fftw/api/f77funcs.h:200-218, 294-312: dfftw_plan_dft_{r2c,c2r}_{1d,2d,3d}_
  * followed by dfftw_execute_, FFTW_ESTIMATE): unnormalised, forward e^{-i},
  * halved axis = Fortran first index.  c2r runs the complex passes first and the
  * x pass last, reading only the real part of the kx = 0 and kx = Nx/2 entries
  * (lib/fftw/rdft/rank-geq2-rdft2.c:40-68).  Unlike FFTW, c2r leaves `in` intact.
  */
+int tara_fft_r2c_1d(int nx, const double* in, double* out);   /* nx 16..8192 */
+int tara_fft_c2r_1d(int nx, const double* in, double* out);
 int tara_fft_r2c_2d(int nx, int ny, const double* in, double* out);
 int tara_fft_c2r_2d(int nx, int ny, const double* in, double* out);
 int tara_fft_r2c_3d(int nx, int ny, int nz, const double* in, double* out);
@@ -86,6 +88,27 @@ int tara_csh_kernel_stats(tara_csh* ctx, int nclass, long* counts, double* ms);
 int tara_csh_num_xshapes(void);
 int tara_csh_tune(tara_csh* ctx, int reps, int* chosen, double* ms);
 int tara_csh_set_xshape(tara_csh* ctx, int shape);
+
+/* Per-step work of the reference's driver on the resident state
+ * (src/main2csh.f95:128-178): real-space omega, ux, uy (each (Nx, Ny), divided
+ * by Nx*Ny as the driver does; any pointer may be NULL).  tara_csh_record
+ * advances nframes x every steps and stores omegak after every `every` steps
+ * (frames: nframes consecutive (Nh, Ny) half spectra). */
+int tara_csh_fields(tara_csh* ctx, double* omega, double* ux, double* uy);
+int tara_csh_record(tara_csh* ctx, int nframes, int every, double* frames);
+/* Initial conditions generated on the device into the context's state (the AB2
+ * history is zeroed).  tara_csh_ic_shear: the shear layer of
+ * src/main2csh.f95:84-110 (box and d as input.ini gives them, multiplied by pi
+ * inside).  tara_csh_ic_random: random-phase omega with
+ * E(k) ~ k^4 exp(-(k/k0)^2), kinetic energy `energy`, Nyquist modes and k = 0
+ * zero, Hermitian kx = 0 column; the phase of mode (i, j) is 2 pi u with u built
+ * from the first two words of Philox-4x32-10(counter = {i + j*Nh, 0, 0, 0},
+ * key = {seed, 0}).  No counterpart in the reference (its driver draws no
+ * random fields). */
+int tara_csh_ic_shear(tara_csh* ctx, double lx_in, double ly_in, double w0, double m0, double d_in);
+int tara_csh_ic_random(tara_csh* ctx, unsigned seed, double k0, double energy);
+/* The counter-based generator used above (host evaluation). */
+int tara_philox4x32_10(const unsigned ctr[4], const unsigned key[2], unsigned out[4]);
 
 /* ------------------------------------------------ batched ensemble (csh)
  * No counterpart in the reference (one run per process there): nruns

@@ -53,6 +53,8 @@ def lib():
         _orc.orc_derive2css.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
         _orc.orc_rk4css_steps.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [C.c_int, _dp]
         _orc.orc_ic2csh.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5 + [_dp, _dp]
+        _orc.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
+        _orc.orc_ic_random.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_double, C.c_double, _dp]
         _orc.orc_derive2coh.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _orc.orc_derive2com.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _orc.orc_derive3com.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -237,6 +239,20 @@ def ref_rk4csh_steps(w, nx, nu, dt, nsteps):
     w = np.array(w, dtype=np.complex128, order="F")
     ref().ref_rk4csh_steps(nx, w.shape[1], nu, dt, nsteps, _ptr(w))
     return w
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint * 4)(*ctr)
+    k = (C.c_uint * 2)(*key)
+    o = (C.c_uint * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def ic_random(nx, ny, seed, k0=4.0, energy=1.0):
+    wk = zeros_spec(nx, ny)
+    lib().orc_ic_random(nx, ny, seed, k0, energy, _ptr(wk))
+    return wk
 
 
 class CohParams(C.Structure):

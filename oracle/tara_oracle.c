@@ -577,6 +577,61 @@ void orc_derive2com(int nx, int ny, const orc_com_params *prm, const double *con
     free(a); free(b); free(duxdx); free(duydy);
 }
 
+/* Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+   easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies by
+   0xD2511F53 and 0xCD9E8D57 with a Weyl key schedule (0x9E3779B9, 0xBB67AE85). */
+void orc_philox4x32_10(const unsigned ctr[4], const unsigned key[2], unsigned out[4])
+{
+    unsigned c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
+        unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
+        unsigned n0 = (unsigned)(p1 >> 32) ^ c1 ^ k0, n1 = (unsigned)p1;
+        unsigned n2 = (unsigned)(p0 >> 32) ^ c3 ^ k1, n3 = (unsigned)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Random-phase vorticity half spectrum (Fortran layout (Nh, Ny)): amplitude
+   sqrt(k E(k)/pi), E(k) = k^4 exp(-(k/k0)^2), scaled to kinetic energy
+   `energy`; phase 2 pi u, u = 53-bit uniform from the first two Philox words at
+   counter {i + j*Nh, 0, 0, 0}, key {seed, 0}; Nyquist row/column and k = 0 zero;
+   kx = 0 column Hermitian.  (The device generator this checks has no
+   counterpart in the reference.) */
+void orc_ic_random(int nx, int ny, unsigned seed, double k0, double energy, double *wk)
+{
+    const double pi = 3.14159265358979323846;
+    int nh = nx / 2 + 1;
+    double s = 0.0;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nh; ++i) {
+            size_t p = (size_t)j * nh + i;
+            RE(wk, p) = 0.0; IM(wk, p) = 0.0;
+            int jj = j, conj = 0;
+            if (i == 0 && j > ny / 2) { jj = ny - j; conj = 1; }
+            if (i == nh - 1 || j == ny / 2 || (i == 0 && jj == 0)) continue;
+            double kx = i, ky = jj < ny / 2 ? jj : jj - ny;
+            double k = sqrt(kx * kx + ky * ky);
+            double e = k * k * k * k * exp(-(k / k0) * (k / k0));
+            double amp = sqrt(k * e / pi);
+            unsigned ctr[4] = {(unsigned)(i + (size_t)jj * nh), 0, 0, 0}, key[2] = {seed, 0}, r[4];
+            orc_philox4x32_10(ctr, key, r);
+            double u = ((double)(r[0] >> 5) * 67108864.0 + (double)(r[1] >> 6)) / 9007199254740992.0;
+            double ph = 2.0 * pi * u;
+            RE(wk, p) = amp * cos(ph);
+            IM(wk, p) = conj ? -amp * sin(ph) : amp * sin(ph);
+            if (i < nh - 1) s += (i == 0 ? 1.0 : 2.0) * amp * amp / (k * k);
+        }
+    /* kinetic energy of the unscaled field: the kx = 0 column above counted
+       both +ky and -ky once each, the other columns twice */
+    double ke = 0.5 * s / ((double)nx * ny * (double)nx * ny);
+    double scale = ke > 0 ? sqrt(energy / ke) : 0.0;
+    for (size_t p = 0; p < 2 * (size_t)nh * ny; ++p) wk[p] *= scale;
+}
+
 /* Shear-layer IC of src/main2csh.f95:84-110 (x-fastest real field, then r2c). */
 void orc_ic2csh(int nx, int ny, double lx_in, double ly_in, double w0, double m0,
                 double d_in, double *omega, double *omegak)

@@ -33,6 +33,8 @@ void orc_rk4csh_steps(int nx, int ny, double nu, double dt, int nsteps, double *
 void orc_derive2css(int nx, int ny, double alpha, double nu, double lambda, const double *omegak, double *dt_new);
 void orc_rk4css_steps(int nx, int ny, double alpha, double nu, double lambda, double dt, int nsteps,
                       double *omegak);
+void orc_philox4x32_10(const unsigned ctr[4], const unsigned key[2], unsigned out[4]);
+void orc_ic_random(int nx, int ny, unsigned seed, double k0, double energy, double *omegak);
 void orc_ic2csh(int nx, int ny, double lx_in, double ly_in, double w0, double m0,
                 double d_in, double *omega, double *omegak);
 

@@ -62,6 +62,8 @@ def lib():
             raise TaraError(f"{LIB_PATH} not built: run `python -m tara_amd.build`")
         L = C.CDLL(LIB_PATH)
         L.tara_last_error.restype = C.c_char_p
+        for n in ("tara_fft_r2c_1d", "tara_fft_c2r_1d"):
+            getattr(L, n).argtypes = [C.c_int, _dp, _dp]
         for n in ("tara_fft_r2c_2d", "tara_fft_c2r_2d"):
             getattr(L, n).argtypes = [C.c_int, C.c_int, _dp, _dp]
         for n in ("tara_fft_r2c_3d", "tara_fft_c2r_3d"):
@@ -81,6 +83,11 @@ def lib():
         L.tara_csh_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.tara_csh_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
         L.tara_csh_num_xshapes.argtypes = []
+        L.tara_csh_fields.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.tara_csh_record.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.tara_csh_ic_shear.argtypes = [C.c_void_p] + [C.c_double] * 5
+        L.tara_csh_ic_random.argtypes = [C.c_void_p, C.c_uint, C.c_double, C.c_double]
+        L.tara_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
         L.tara_csh_tune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]
         L.tara_csh_set_xshape.argtypes = [C.c_void_p, C.c_int]
         L.tara_ens_create.argtypes = [C.c_int, C.POINTER(C.c_int), _dp, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
@@ -267,6 +274,26 @@ class CshSolver:
     def set_xshape(self, shape):
         _check(lib().tara_csh_set_xshape(self._h, shape))
 
+    def fields(self):
+        """Real-space (omega, ux, uy), each (Nx, Ny), as src/main2csh.f95:128-178 forms them."""
+        out = [np.zeros((self.nx, self.ny), order="F") for _ in range(3)]
+        _check(lib().tara_csh_fields(self._h, *[_p(a) for a in out]))
+        return out
+
+    def record(self, nframes, every):
+        """Advances nframes * every steps; returns omegak after every `every` steps."""
+        frames = np.zeros((nframes, self.ny, self.nh), np.complex128)
+        _check(lib().tara_csh_record(self._h, nframes, every, frames.ctypes.data_as(_dp)))
+        return [np.asfortranarray(f.T) for f in frames]
+
+    def ic_shear(self, lx_in=2.0, ly_in=2.0, w0=25.0, m0=2.0, d_in=3.0 / 128.0):
+        """The reference's shear layer generated on the device (history zeroed)."""
+        _check(lib().tara_csh_ic_shear(self._h, lx_in, ly_in, w0, m0, d_in))
+
+    def ic_random(self, seed, k0=4.0, energy=1.0):
+        """Philox random-phase vorticity generated on the device (history zeroed)."""
+        _check(lib().tara_csh_ic_random(self._h, seed, k0, energy))
+
     def close(self):
         if self._h:
             lib().tara_csh_destroy(self._h)
@@ -277,6 +304,15 @@ class CshSolver:
             self.close()
         except Exception:
             pass
+
+
+def philox4x32_10(ctr, key):
+    """Host evaluation of the generator behind CshSolver.ic_random."""
+    c = (C.c_uint * 4)(*ctr)
+    k = (C.c_uint * 2)(*key)
+    o = (C.c_uint * 4)()
+    _check(lib().tara_philox4x32_10(c, k, o))
+    return list(o)
 
 
 class Ensemble:

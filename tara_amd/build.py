@@ -55,6 +55,17 @@ def build_hip(jobs=None):
     return LIB
 
 
+def build_shim():
+    """libtara_dfftw.so: FFTW's Fortran entry points over libtara_hip.so (host C++)."""
+    out = os.path.join(HERE, "libtara_dfftw.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(HERE, "shim", "dfftw_shim.cpp"), "-o", tmp, "-L" + HERE, "-ltara_hip",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_fortran():
     """ISO_C_BINDING wrappers keeping the reference's subroutine names
     (libtara_fortran.so), and the Fortran test drivers under tests/fortran/."""
@@ -76,14 +87,16 @@ def build_fortran():
     tdir = os.path.join(ROOT, "tests", "fortran")
     for s in sorted(f for f in os.listdir(tdir) if f.endswith(".f90")):
         exe = os.path.join(tdir, s[:-4])
-        _run([fc, "-O2", "-J" + OBJ, os.path.join(tdir, s), "-o", exe + ".tmp", "-L" + HERE,
-              "-ltara_fortran", "-ltara_hip", "-Wl,-rpath,$ORIGIN/../../tara_amd"])
+        libs = ["-ltara_dfftw"] if s.startswith("drive_dfftw") else ["-ltara_fortran"]
+        _run([fc, "-O2", "-J" + OBJ, os.path.join(tdir, s), "-o", exe + ".tmp", "-L" + HERE] + libs +
+             ["-ltara_hip", "-Wl,-rpath,$ORIGIN/../../tara_amd"])
         os.replace(exe + ".tmp", exe)
     return FLIB
 
 
 def main():
     build_hip()
+    build_shim()
     build_fortran()
     print(LIB)
 

@@ -256,6 +256,25 @@ extern "C" int tara_css_create(int nx, int ny, double alpha, double nu, double l
     return create_ctx(nx, ny, css_phys(alpha, nu, lambda), dt, scheme, out);
 }
 
+namespace tara {
+int csh_ctx_state(tara_csh* c, double2** w, const CshPlan** p, hipStream_t* st)
+{
+    TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    *w = c->w.as<double2>();
+    *p = &c->plan;
+    *st = c->st;
+    return TARA_OK;
+}
+
+int csh_ctx_zero_history(tara_csh* c)
+{
+    TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    TARA_HIP(hipMemsetAsync(c->aux.p, 0, c->aux.bytes, c->st));
+    TARA_HIP(hipStreamSynchronize(c->st));
+    return TARA_OK;
+}
+}  // namespace tara
+
 extern "C" int tara_csh_upload(tara_csh* c, const double* omegak)
 {
     TARA_CHECK(c && omegak, TARA_ERR_ARG, "null argument");

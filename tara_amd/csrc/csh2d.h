@@ -70,6 +70,10 @@ int csh_default_xshape(int nx);
 int csh_num_xshapes();
 int csh_xshape_ok(const CshPlan& p, int c);
 int csh_xpass(const CshPlan& p, hipStream_t st);
+// Access to a tara_csh context's device state (capi_csh.hip): the ky-major
+// omega_hat, the plan and the stream; and zeroing of its AB2 history.
+int csh_ctx_state(tara_csh* c, double2** w, const CshPlan** p, hipStream_t* st);
+int csh_ctx_zero_history(tara_csh* c);
 int launch_ab2(const double2* w, double2* wn, const double2* fo, const double2* fn, double dt, size_t n,
                hipStream_t st);
 

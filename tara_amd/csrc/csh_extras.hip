@@ -1,0 +1,243 @@
+// Work of the 2D time loop that the reference's driver does on the host
+// every step, moved onto the device state of a tara_csh context:
+//  - real-space omega, ux, uy (src/main2csh.f95:128-178: psi, u spectra, three
+//    c2r, division by Nx*Ny), for snapshots and tracers at the caller's cadence;
+//  - initial conditions: the reference's shear layer (src/main2csh.f95:84-110)
+//    and a seeded random-phase field whose phases come from the counter-based
+//    Philox-4x32-10 generator (Salmon, Moraes, Dror, Shaw, SC'11), so that any
+//    mode's phase depends only on (seed, mode index), independent of the launch.
+#include <cmath>
+#include <vector>
+
+#include "csh2d.h"
+#include "fft_api.h"
+#include "layout.h"
+#include "tara_hip.h"
+
+namespace tara {
+
+// ------------------------------------------------------------------ Philox
+// Philox-4x32 with 10 rounds: multipliers 0xD2511F53, 0xCD9E8D57, Weyl key
+// increments 0x9E3779B9, 0xBB67AE85 (the published constants of the method).
+struct U4 {
+    unsigned x, y, z, w;
+};
+
+__host__ __device__ inline void mulhilo(unsigned a, unsigned b, unsigned* hi, unsigned* lo)
+{
+    const unsigned long long p = (unsigned long long)a * b;
+    *hi = (unsigned)(p >> 32);
+    *lo = (unsigned)p;
+}
+
+__host__ __device__ inline U4 philox4x32_10(U4 c, unsigned k0, unsigned k1)
+{
+    for (int r = 0; r < 10; ++r) {
+        unsigned h0, l0, h1, l1;
+        mulhilo(0xD2511F53u, c.x, &h0, &l0);
+        mulhilo(0xCD9E8D57u, c.z, &h1, &l1);
+        c = U4{h1 ^ c.y ^ k0, l1, h0 ^ c.w ^ k1, l0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Uniform in [0, 1) with 53 bits from two 32-bit words.
+__host__ __device__ inline double u53(unsigned a, unsigned b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// ------------------------------------------------------------------ kernels
+// Spectra of omega, ux = +i ky psi, uy = -i kx psi (psi = omega/k^2, psi(0,0) = 0)
+// from the ky-major state into three padded kx-fastest planes (pitch ld).
+__global__ void k_fields_spec(const double2* w, int nh, int ny, int ld, double2* s)
+{
+    const size_t plane = (size_t)ld * ny;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)nh * ny;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q % ny), i = (int)(q / ny);
+        const double2 z = w[q];
+        const double kx = (double)i, ky = (double)(j < ny / 2 ? j : j - ny);
+        const double k2 = kx * kx + ky * ky;
+        const double2 ps = k2 != 0.0 ? make_double2(z.x / k2, z.y / k2) : make_double2(0.0, 0.0);
+        const size_t o = (size_t)j * ld + i;
+        s[o] = z;
+        s[plane + o] = make_double2(-(ky * ps.y), ky * ps.x);
+        s[2 * plane + o] = make_double2(kx * ps.y, -(kx * ps.x));
+    }
+}
+
+// Shear layer of src/main2csh.f95:94-106 on the (Nx, Ny) grid, x = (i-1) dx.
+__global__ void k_shear_layer(double* out, int nx, int ny, double dx, double dy, double pi, double w0, double m0,
+                              double d)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)nx * ny;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx), j = (int)(q / nx);
+        const double x = (double)i * dx, y = (double)j * dy;
+        const double c1 = cosh((y + 0.5 * pi) / d), c2 = cosh((y - 0.5 * pi) / d);
+        const double c3 = cosh((y + 1.5 * pi) / d), c4 = cosh((y - 1.5 * pi) / d);
+        double v = w0 / (c1 * c1) - w0 / (c2 * c2);
+        v = v - w0 / (c3 * c3) + w0 / (c4 * c4);
+        out[q] = v + 0.10 * cos(m0 * x);
+    }
+}
+
+// Random-phase vorticity on the half plane, written ky-major: amplitude
+// sqrt(k E(k)/pi) with E(k) = k^4 exp(-(k/k0)^2), times scale; phase 2 pi u
+// with u from Philox(counter = (i + j*nh, 0, 0, 0), key = (seed, 0)).  Modes on
+// the Nyquist row / column and k = 0 are zero; the kx = 0 column is Hermitian
+// (ky < 0 entries are the conjugates of ky > 0 ones).
+__global__ void k_random_phase(double2* w, int nh, int ny, unsigned seed, double k0, double scale)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)nh * ny;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q % ny), i = (int)(q / ny);
+        int jj = j;
+        bool conj = false;
+        if (i == 0 && j > ny / 2) {
+            jj = ny - j;
+            conj = true;
+        }
+        double2 v = make_double2(0.0, 0.0);
+        if (!(i == nh - 1 || j == ny / 2 || (i == 0 && jj == 0))) {
+            const double kx = (double)i, ky = (double)(jj < ny / 2 ? jj : jj - ny);
+            const double k = sqrt(kx * kx + ky * ky);
+            const double e = k * k * k * k * exp(-(k / k0) * (k / k0));
+            const double amp = sqrt(k * e / 3.14159265358979323846) * scale;
+            const U4 r = philox4x32_10(U4{(unsigned)(i + (size_t)jj * nh), 0u, 0u, 0u}, seed, 0u);
+            const double ph = 2.0 * 3.14159265358979323846 * u53(r.x, r.y);
+            v = make_double2(amp * cos(ph), amp * sin(ph));
+            if (conj) v.y = -v.y;
+        }
+        w[q] = v;
+    }
+}
+
+static unsigned grid_for(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b ? b : 1));
+}
+
+// Kinetic energy 1/2 <|u|^2> of the random-phase field before scaling: Parseval
+// with half-plane weights (2 for kx > 0), |omega_k| = amp, |u_k| = amp / k.
+static double random_phase_energy(int nx, int ny, double k0)
+{
+    const int nh = nx / 2 + 1;
+    double s = 0.0;
+    for (int i = 0; i < nh - 1; ++i)
+        for (int j = 0; j < ny; ++j) {
+            if (j == ny / 2 || (i == 0 && j == 0)) continue;
+            const double kx = i, ky = j < ny / 2 ? j : j - ny;
+            const double k2 = kx * kx + ky * ky, k = std::sqrt(k2);
+            const double e = k2 * k2 * std::exp(-(k / k0) * (k / k0));
+            s += (i == 0 ? 1.0 : 2.0) * (k * e / 3.14159265358979323846) / k2;
+        }
+    return 0.5 * s / ((double)nx * ny * (double)nx * ny);
+}
+
+}  // namespace tara
+
+// ------------------------------------------------------------------ C ABI
+using namespace tara;
+
+extern "C" int tara_csh_fields(tara_csh* c, double* omega, double* ux, double* uy)
+{
+    double2* w = nullptr;
+    const CshPlan* p = nullptr;
+    hipStream_t st = nullptr;
+    int rc = csh_ctx_state(c, &w, &p, &st);
+    if (rc) return rc;
+    const int nx = p->nx, ny = p->ny, ld = spec_ld(nx);
+    TARA_CHECK(nx >= 16 && ny >= 8, TARA_ERR_UNSUPPORTED, "fields: grid too small");
+    DevBuf spec, real;
+    if ((rc = spec.alloc(3 * (size_t)ld * ny * sizeof(double2))) || (rc = real.alloc(3 * (size_t)nx * ny * 8)))
+        return rc;
+    TARA_HIP(hipMemsetAsync(spec.p, 0, spec.bytes, st));
+    hipLaunchKernelGGL(k_fields_spec, dim3(grid_for((size_t)p->nh * ny)), dim3(256), 0, st, w, p->nh, ny, ld,
+                       spec.as<double2>());
+    TARA_HIP(hipGetLastError());
+    if ((rc = col_fft(spec.as<double2>(), ny, ld, ld, (long)ld * ny, 3, +1, st))) return rc;
+    if ((rc = row_c2r(spec.as<double2>(), real.as<double>(), nx, ld, 3 * ny, 1.0 / ((double)nx * ny), st)))
+        return rc;
+    double* outs[3] = {omega, ux, uy};
+    const size_t nr = (size_t)nx * ny;
+    for (int f = 0; f < 3; ++f)
+        if (outs[f]) TARA_HIP(hipMemcpyAsync(outs[f], real.as<double>() + f * nr, nr * 8, hipMemcpyDeviceToHost, st));
+    TARA_HIP(hipStreamSynchronize(st));
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_record(tara_csh* c, int nframes, int every, double* frames)
+{
+    TARA_CHECK(c && frames && nframes >= 0 && every >= 1, TARA_ERR_ARG, "bad argument");
+    double2* w = nullptr;
+    const CshPlan* p = nullptr;
+    hipStream_t st = nullptr;
+    int rc = csh_ctx_state(c, &w, &p, &st);
+    if (rc) return rc;
+    const size_t per = (size_t)p->nh * p->ny * 2;
+    for (int f = 0; f < nframes; ++f) {
+        if ((rc = tara_csh_advance(c, every))) return rc;
+        if ((rc = tara_csh_download(c, frames + f * per))) return rc;
+    }
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_ic_shear(tara_csh* c, double lx_in, double ly_in, double w0, double m0, double d_in)
+{
+    double2* w = nullptr;
+    const CshPlan* p = nullptr;
+    hipStream_t st = nullptr;
+    int rc = csh_ctx_state(c, &w, &p, &st);
+    if (rc) return rc;
+    const int nx = p->nx, ny = p->ny, nh = p->nh, ld = spec_ld(nx);
+    TARA_CHECK(nx >= 16 && ny >= 8, TARA_ERR_UNSUPPORTED, "ic: grid too small");
+    const double pi = 3.14159265358979323846;  // src/main2csh.f95:9
+    const double lx = lx_in * pi, ly = ly_in * pi;
+    DevBuf real, spec, flat;
+    if ((rc = real.alloc((size_t)nx * ny * 8)) || (rc = spec.alloc((size_t)ld * ny * sizeof(double2))) ||
+        (rc = flat.alloc((size_t)nh * ny * sizeof(double2))))
+        return rc;
+    hipLaunchKernelGGL(k_shear_layer, dim3(grid_for((size_t)nx * ny)), dim3(256), 0, st, real.as<double>(), nx, ny,
+                       lx / nx, ly / ny, pi, w0, m0, d_in * pi);
+    TARA_HIP(hipGetLastError());
+    if ((rc = row_r2c(real.as<double>(), spec.as<double2>(), nx, ld, ny, st))) return rc;
+    if ((rc = col_fft(spec.as<double2>(), ny, ld, ld, (long)ld * ny, 1, -1, st))) return rc;
+    TARA_HIP(hipMemcpy2DAsync(flat.p, nh * sizeof(double2), spec.p, ld * sizeof(double2), nh * sizeof(double2), ny,
+                              hipMemcpyDeviceToDevice, st));
+    if ((rc = to_kymajor(flat.as<double2>(), w, nh, ny, 1, st))) return rc;
+    TARA_HIP(hipStreamSynchronize(st));
+    return csh_ctx_zero_history(c);
+}
+
+extern "C" int tara_csh_ic_random(tara_csh* c, unsigned seed, double k0, double energy)
+{
+    double2* w = nullptr;
+    const CshPlan* p = nullptr;
+    hipStream_t st = nullptr;
+    int rc = csh_ctx_state(c, &w, &p, &st);
+    if (rc) return rc;
+    TARA_CHECK(k0 > 0 && energy >= 0, TARA_ERR_ARG, "ic: k0 must be positive, energy non-negative");
+    const double e0 = random_phase_energy(p->nx, p->ny, k0);
+    const double scale = e0 > 0 ? std::sqrt(energy / e0) : 0.0;
+    hipLaunchKernelGGL(k_random_phase, dim3(grid_for((size_t)p->nh * p->ny)), dim3(256), 0, st, w, p->nh, p->ny,
+                       seed, k0, scale);
+    TARA_HIP(hipGetLastError());
+    TARA_HIP(hipStreamSynchronize(st));
+    return csh_ctx_zero_history(c);
+}
+
+extern "C" int tara_philox4x32_10(const unsigned ctr[4], const unsigned key[2], unsigned out[4])
+{
+    TARA_CHECK(ctr && key && out, TARA_ERR_ARG, "null argument");
+    const U4 r = philox4x32_10(U4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+    return TARA_OK;
+}

@@ -173,39 +173,43 @@ using namespace tara;
 namespace {
 int check_dims(int nx, int ny, int nz)
 {
-    TARA_CHECK(is_pow2(nx) && is_pow2(ny) && (nz == 1 || is_pow2(nz)) && nx >= 16 && nx <= 8192 && ny >= 8 &&
-                   ny <= 4096 && nz <= 4096,
-               TARA_ERR_UNSUPPORTED, "transform size %dx%dx%d unsupported (powers of two; nx 16..8192, ny, nz 8..4096)",
-               nx, ny, nz);
+    const bool one_d = ny == 1 && nz == 1;
+    TARA_CHECK(is_pow2(nx) && nx >= 16 && nx <= 8192 &&
+                   (one_d || (is_pow2(ny) && (nz == 1 || is_pow2(nz)) && ny >= 8 && ny <= 4096 && nz <= 4096)),
+               TARA_ERR_UNSUPPORTED,
+               "transform size %dx%dx%d unsupported (powers of two; nx 16..8192, ny, nz 8..4096 or both 1)", nx, ny,
+               nz);
     return TARA_OK;
 }
 
 // Host-array transform: copy in (host layout pitch nh -> device pitch ld), run, copy out.
+// A 1-D transform runs as row 0 of a block of 8 rows (the row kernels' unit).
 int fft_host(int nx, int ny, int nz, const double* in, double* out, bool forward)
 {
     int rc = check_dims(nx, ny, nz);
     if (rc) return rc;
     const int nh = nx / 2 + 1, ld = spec_ld(nx);
-    const size_t rows = (size_t)ny * nz;
+    const size_t rows = (size_t)ny * nz, drows = rows < 8 ? 8 : rows;
     DevBuf spec, real;
-    if ((rc = spec.alloc(rows * ld * sizeof(double2)))) return rc;
-    if ((rc = real.alloc(rows * nx * sizeof(double)))) return rc;
+    if ((rc = spec.alloc(drows * ld * sizeof(double2)))) return rc;
+    if ((rc = real.alloc(drows * nx * sizeof(double)))) return rc;
     hipStream_t st = nullptr;
     double2* S = spec.as<double2>();
     if (forward) {
+        if (drows > rows) TARA_HIP(hipMemset(real.p, 0, real.bytes));
         TARA_HIP(hipMemcpy(real.p, in, rows * nx * sizeof(double), hipMemcpyHostToDevice));
-        if ((rc = row_r2c(real.as<double>(), S, nx, ld, (int)rows, st))) return rc;
-        if ((rc = col_fft(S, ny, ld, ld, (long)ld * ny, nz, -1, st))) return rc;
+        if ((rc = row_r2c(real.as<double>(), S, nx, ld, (int)drows, st))) return rc;
+        if (ny > 1 && (rc = col_fft(S, ny, ld, ld, (long)ld * ny, nz, -1, st))) return rc;
         if (nz > 1 && (rc = col_fft(S, nz, ld, (long)ld * ny, ld, ny, -1, st))) return rc;
         TARA_HIP(hipMemcpy2D(out, nh * sizeof(double2), S, ld * sizeof(double2), nh * sizeof(double2), rows,
                              hipMemcpyDeviceToHost));
     } else {
-        TARA_HIP(hipMemset(S, 0, rows * ld * sizeof(double2)));
+        TARA_HIP(hipMemset(S, 0, drows * ld * sizeof(double2)));
         TARA_HIP(hipMemcpy2D(S, ld * sizeof(double2), in, nh * sizeof(double2), nh * sizeof(double2), rows,
                              hipMemcpyHostToDevice));
         if (nz > 1 && (rc = col_fft(S, nz, ld, (long)ld * ny, ld, ny, +1, st))) return rc;
-        if ((rc = col_fft(S, ny, ld, ld, (long)ld * ny, nz, +1, st))) return rc;
-        if ((rc = row_c2r(S, real.as<double>(), nx, ld, (int)rows, 1.0, st))) return rc;
+        if (ny > 1 && (rc = col_fft(S, ny, ld, ld, (long)ld * ny, nz, +1, st))) return rc;
+        if ((rc = row_c2r(S, real.as<double>(), nx, ld, (int)drows, 1.0, st))) return rc;
         TARA_HIP(hipMemcpy(out, real.p, rows * nx * sizeof(double), hipMemcpyDeviceToHost));
     }
     TARA_HIP(hipDeviceSynchronize());
@@ -213,6 +217,16 @@ int fft_host(int nx, int ny, int nz, const double* in, double* out, bool forward
 }
 }  // namespace
 
+extern "C" int tara_fft_r2c_1d(int nx, const double* in, double* out)
+{
+    TARA_CHECK(in && out, TARA_ERR_ARG, "null pointer");
+    return fft_host(nx, 1, 1, in, out, true);
+}
+extern "C" int tara_fft_c2r_1d(int nx, const double* in, double* out)
+{
+    TARA_CHECK(in && out, TARA_ERR_ARG, "null pointer");
+    return fft_host(nx, 1, 1, in, out, false);
+}
 extern "C" int tara_fft_r2c_2d(int nx, int ny, const double* in, double* out)
 {
     TARA_CHECK(in && out, TARA_ERR_ARG, "null pointer");

@@ -47,3 +47,11 @@ def test_argument_errors_without_gpu(lib):
     rc = lib.tara_csh_create(64, 64, ctypes.c_double(0.01), ctypes.c_double(1e-4), 7, ctypes.byref(ctypes.c_void_p()))
     assert rc == -1
     assert b"scheme" in lib.tara_last_error()
+
+
+def test_host_philox_matches_known_answers():
+    """tara_philox4x32_10 is host code: it runs without a GPU."""
+    import tara_amd as T
+    from test_oracle import PHILOX_KAT
+    for ctr, key, want in PHILOX_KAT:
+        assert T.philox4x32_10(ctr, key) == want

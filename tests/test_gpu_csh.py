@@ -210,3 +210,14 @@ def test_tune_picks_a_running_shape():
     chosen, ms = s.tune(reps=2)
     assert ms[chosen] > 0 and ms[chosen] == min(m for m in ms if m > 0)
     s.close()
+
+
+@pytest.mark.parametrize("n", [16, 64, 4096])
+def test_fft_1d_vs_numpy(n):
+    x = np.random.default_rng(n).standard_normal(n)
+    X = np.zeros(n // 2 + 1, np.complex128)
+    T.lib().tara_fft_r2c_1d(n, x.ctypes.data_as(T._dp), X.ctypes.data_as(T._dp))
+    assert rel_l2(X, np.fft.rfft(x)) < 1e-14
+    y = np.zeros(n)
+    T.lib().tara_fft_c2r_1d(n, X.ctypes.data_as(T._dp), y.ctypes.data_as(T._dp))
+    assert rel_l2(y, n * x) < 1e-14

@@ -322,3 +322,33 @@ def test_coh_steppers_closed_forms(golden_fam2d):
     dt = 1e-3
     assert np.array_equal(golden_fam2d["pt_rk4nk"], w + (dt / 6.0) * (((s + 2.0 * s) + 2.0 * s) + s))
     assert np.array_equal(golden_fam2d["pt_pcnk"], w + ((s + s) * dt) / 2.0)
+
+
+# ------------------------------------------------- counter-based initial conditions
+PHILOX_KAT = [  # Philox-4x32-10 known-answer vectors published with the method
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert O.philox4x32_10(ctr, key) == want
+
+
+def test_ic_random_properties():
+    """Unit kinetic energy (from the real-space velocity), zero mean, Nyquist
+    modes zero, and the spectrum a real field's (c2r then r2c returns it)."""
+    n = 64
+    wk = O.ic_random(n, n, 7, 4.0, 1.0)
+    kx = np.arange(n // 2 + 1)[:, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :]
+    k2 = np.where(kx * kx + ky * ky > 0, kx * kx + ky * ky, 1.0)
+    ux = O.c2r(1j * ky * wk / k2, n) / n**2
+    uy = O.c2r(-1j * kx * wk / k2, n) / n**2
+    assert abs(0.5 * np.mean(ux * ux + uy * uy) - 1.0) < 1e-12
+    assert wk[0, 0] == 0 and np.all(wk[n // 2, :] == 0) and np.all(wk[:, n // 2] == 0)
+    assert rel_l2(O.r2c(O.c2r(wk, n)) / n**2, wk) < 1e-14
+    assert not np.array_equal(wk, O.ic_random(n, n, 8, 4.0, 1.0))
