@@ -274,6 +274,33 @@ int tara_mhd_destroy(tara_mhd* ctx);
 int tara_mhd_set_timing(tara_mhd* ctx, int on);
 int tara_mhd_kernel_stats(tara_mhd* ctx, int nclass, long* counts, double* ms);
 
+/* ------------------------------------------------------ snapshots and data
+ * Host I/O, no GPU needed.
+ * Checkpoints (FP64): a 64-byte header ("TARACKPT", version 1, kind 1, nx, ny,
+ * nz, nfields as uint32, step as int64, time as double, 16 reserved bytes), then
+ * nfields (Nh, Ny, Nz) complex(8) half spectra in the reference layout (nz = 1
+ * for 2D).
+ * tara_fort_write: the reference's text snapshot (src/main2csh.f95:170-178):
+ * per grid point, i outer and j inner, one line "x y omega ux uy" with
+ * x = (i-1) Lx/Nx, y = (j-1) Ly/Ny; values printed %24.16E (list-directed
+ * output's spacing is compiler-specific; the columns are the reference's).
+ * Trajectory pairs for surrogate training: a 64-byte header ("TARAPAIR",
+ * version 1, dtype 1 = float32 / 2 = bfloat16, nx, ny as uint32, npairs as
+ * uint64, dt between the fields of a pair, nu, 16 reserved bytes), then per pair
+ * the real-space (Nx, Ny) omega before and after, Fortran order.  BF16 values are
+ * rounded to nearest even from the FP32 value.  No counterpart in the reference.
+ */
+int tara_ckpt_write(const char* path, int nx, int ny, int nz, int nfields, long step, double time,
+                    const double* const fields[]);
+int tara_ckpt_read_header(const char* path, int* nx, int* ny, int* nz, int* nfields, long* step, double* time);
+int tara_ckpt_read(const char* path, double* const fields[]);
+int tara_fort_write(const char* path, int nx, int ny, double lx, double ly, const double* omega, const double* ux,
+                    const double* uy);
+typedef struct tara_pairs tara_pairs;
+int tara_pairs_create(const char* path, int nx, int ny, int dtype, double dt_pair, double nu, tara_pairs** out);
+int tara_pairs_append(tara_pairs* p, const double* omega_in, const double* omega_out);
+int tara_pairs_close(tara_pairs* p);
+
 #ifdef __cplusplus
 }
 #endif

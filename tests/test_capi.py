@@ -1,10 +1,12 @@
 """CPU-side checks of the C ABI: the library loads and exports exactly what
-include/tara_hip.h declares.  No compute calls (no GPU here)."""
+include/tara_hip.h declares; host-only entry points (I/O, the counter-based
+generator) run here.  No GPU compute calls (no GPU here)."""
 import ctypes
 import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -55,3 +57,51 @@ def test_host_philox_matches_known_answers():
     from test_oracle import PHILOX_KAT
     for ctr, key, want in PHILOX_KAT:
         assert T.philox4x32_10(ctr, key) == want
+
+
+def test_checkpoint_roundtrip(tmp_path):
+    """Host I/O: runs without a GPU."""
+    from tara_amd import snapio
+    rng = np.random.default_rng(0)
+    f = [np.asfortranarray(rng.standard_normal((9, 16)) + 1j * rng.standard_normal((9, 16))) for _ in range(3)]
+    snapio.write_checkpoint(tmp_path / "c.bin", f, 16, step=42, time=0.5)
+    g, h = snapio.read_checkpoint(tmp_path / "c.bin")
+    assert h == {"nx": 16, "ny": 16, "nz": 1, "step": 42, "time": 0.5}
+    assert all(np.array_equal(a, b) for a, b in zip(f, g))
+    f3 = [np.asfortranarray(rng.standard_normal((5, 8, 4)) + 0j) for _ in range(2)]
+    snapio.write_checkpoint(tmp_path / "d.bin", f3, 8)
+    g3, h3 = snapio.read_checkpoint(tmp_path / "d.bin")
+    assert h3["nz"] == 4 and all(np.array_equal(a, b) for a, b in zip(f3, g3))
+
+
+def test_pair_store_fp32_and_bf16(tmp_path):
+    from tara_amd import snapio
+    rng = np.random.default_rng(1)
+    a = [np.asfortranarray(rng.standard_normal((16, 8))) for _ in range(6)]
+    for dt, tol in ((snapio.FP32, 1e-7), (snapio.BF16, 2.0 ** -8)):
+        w = snapio.PairWriter(tmp_path / f"p{dt}.bin", 16, 8, dt, dt_pair=0.05, nu=1e-3)
+        for k in range(3):
+            w.append(a[2 * k], a[2 * k + 1])
+        w.close()
+        x, y, h = snapio.read_pairs(tmp_path / f"p{dt}.bin")
+        assert h["npairs"] == 3 and h["dt_pair"] == 0.05 and x.shape == (3, 16, 8)
+        for k in range(3):
+            assert np.max(np.abs(x[k] - a[2 * k]) / (np.abs(a[2 * k]) + 1e-30)) <= tol
+            assert np.max(np.abs(y[k] - a[2 * k + 1]) / (np.abs(a[2 * k + 1]) + 1e-30)) <= tol
+    # bf16 rounding is to nearest even: 1 + 2^-8 (a tie) rounds down to 1, 1 + 3 * 2^-8 up
+    w = snapio.PairWriter(tmp_path / "t.bin", 2, 1, snapio.BF16)
+    w.append(np.array([[1 + 2.0 ** -8], [1 + 3 * 2.0 ** -8]]), np.zeros((2, 1)))
+    w.close()
+    x, _, _ = snapio.read_pairs(tmp_path / "t.bin")
+    assert x[0, 0, 0] == 1.0 and x[0, 1, 0] == 1 + 2.0 ** -6
+
+
+def test_fort_text_snapshot(tmp_path):
+    from tara_amd import snapio
+    om = np.asfortranarray(np.arange(8.0).reshape(4, 2))
+    snapio.write_fort(tmp_path / "fort.00000100", om, 2 * om, 3 * om, 2 * np.pi, 2 * np.pi)
+    rows = np.loadtxt(tmp_path / "fort.00000100")
+    assert rows.shape == (8, 5)
+    # i outer, j inner (src/main2csh.f95:170-178)
+    assert np.allclose(rows[1], [0.0, np.pi, om[0, 1], 2 * om[0, 1], 3 * om[0, 1]])
+    assert np.allclose(rows[2, :2], [np.pi / 2, 0.0])
