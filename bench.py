@@ -30,10 +30,19 @@ HBM_PEAK_GBS = 8000.0
 # --no-cpu-baseline), each scaled by what the counter reads for a device copy
 # of known size (tara_amd/calib/copy_probe: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
 MEASURED_TRAFFIC = {
+    ("k_csh_yinv", 4096): 6.7951e8,
+    ("k_csh_yfwd", 4096): 6.7412e8,      # mean over the four RK4 stage variants
+    # k_csh_x per x-pass shape (the autotuner's choice, see tara_csh_tune)
+    ("k_csh_x", 4096, 0): 7.4397e8,
+    ("k_csh_x", 4096, 1): 7.0603e8,
+    ("k_csh_x", 4096, 2): 7.0697e8,
+    ("k_csh_x", 4096, 3): 2.1826e9,
+    ("k_csh_x", 4096, 4): 6.9727e8,
+    ("k_csh_x", 4096, 5): 6.8065e8,
     ("k_mhd_col_inverse_zy", 256): 3.9932e9,   # two launches (z, then y), 1.9966e9 each
-    ("k_mhd_x", 256): 2.9069e9,
+    ("k_mhd_x", 256): 2.9070e9,
     ("k_mhd_col_forward", 256): 3.9931e9,
-    ("k_mhd_zfwd", 256): 6.4691e9,
+    ("k_mhd_zfwd", 256): 6.8447e9,
 }
 METRIC = "grid-pt steps/s FP64 (2D 4096², 3D 256³ MHD); FNO sim-time/s at ≤5% L2 err"
 
@@ -227,7 +236,7 @@ def main():
                    "ic": "random-phase E(k)~k^4 exp(-(k/4)^2), unit KE, seed 1"},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((names[dom], n)),
+                     "traffic": MEASURED_TRAFFIC.get((names[dom], n, xshape) if dom == 1 else (names[dom], n)),
                      "algorithmic_bytes_per_launch": model[dom]},
         "kernels": kernels,
         "xpass_shape": xshape,

@@ -7,11 +7,11 @@
 
 namespace tara {
 
-template <int N>
+template <int N, int EM = 8>
 struct CShape {
-    static constexpr int T = RF<N>::T;        // threads per column
-    static constexpr int NT = 8 * T;          // 8 kx columns per tile (128-byte rows)
-    static constexpr int LP = RF<N>::LDS + 1; // per-column buffer; +1 shifts banks between columns
+    static constexpr int T = RF<N, EM>::T;        // threads per column
+    static constexpr int NT = 8 * T;              // 8 kx columns per tile (128-byte rows)
+    static constexpr int LP = RF<N, EM>::LDS + 1; // per-column buffer; +1 shifts banks between columns
 };
 
 struct ColArgs {

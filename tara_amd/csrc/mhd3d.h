@@ -21,6 +21,7 @@ struct MhdPlan {
     DevBuf kx, ky, kz, imin;
     const double2* tw_y = nullptr;
     const double2* tw_z = nullptr;
+    const double2* tw_z4 = nullptr;  // radix cap 4 (z forward pass at 4 elements per thread)
     const double2* tw_x = nullptr;  // length nx, radix cap 4 (x pass)
 };
 
