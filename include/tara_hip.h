@@ -208,7 +208,7 @@ typedef struct {
  * src/main2com.f95:231-233 (the routine's own x, y are uninitialised,
  * src/derive2com.f95:15).  curl B is formed as the reference writes it
  * (from the velocity derivatives divided by Nx Ny once more,
- * src/derive2com.f95:172-175). */
+ * src/derive2com.f95:186-187). */
 int tara_derive2com(int nx, int ny, int nh, const tara_com_params* p, const double* const state[6],
                     double* const rhs[6]);
 /* Replaces src/ab2com.f95:6 (AB2 on all six fields). */

@@ -20,13 +20,13 @@
 //      each, which is the same linear map.
 //   com: 1 k_f2_prep   backward y FFT of the 6 state fields -> A
 //        2 k_com_x1    rho, rho u -> u = rho u/rho and the forcing
-//                      (derive2com.f95:100-117), forward x -> C (4 fields)
+//                      (derive2com.f95:88-117), forward x -> C (4 fields)
 //        3 k_com_mid   forward y of u, F -> K (ux_k, uy_k, Fx_k, Fy_k), and the
-//                      backward y of i kx ux_k, i ky uy_k -> D (:142-176)
+//                      backward y of i kx ux_k, i ky uy_k -> D (:124-168)
 //        4 k_com_x2    8 c2r (state + the two derivatives), the fluxes of
-//                      :181-226, 7 distinct forward transforms -> B
+//                      :180-214, 7 distinct forward transforms -> B
 //        5 k_com_yfwd  forward y, i k . flux, k^2 terms, dealias, forcing, assembly
-//                      (:252-420), RHS out or AB2 in place.
+//                      (:255-420), RHS out or AB2 in place.
 #include <cmath>
 #include <vector>
 
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_com_x1(F2XArgs x, const double2* tw)
     for (int e = 0; e < E; ++e) {
         uy[e] = (v[e].x * x.norm) / rho[e];
         const double xi = (double)rf_final_idx<NX, 4>(t, e) * x.p2;
-        // Fx = -rho A sin(kf y), Fy = +rho A sin(kf x)  (derive2com.f95:113-114)
+        // Fx = -rho A sin(kf y), Fy = +rho A sin(kf x)  (derive2com.f95:109-110)
         fx[e] = -(rho[e] * x.p0 * sy);
         fy[e] = rho[e] * x.p0 * sin(x.p1 * xi);
     }
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_com_x2(F2XArgs x, const double2* tw)
     }
     double f0[E], f1[E];
     double2* B0 = x.b + ro;
-    // momentum fluxes (derive2com.f95:181-196)
+    // momentum fluxes (derive2com.f95:196-199)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const double ux = rux[e] / rho[e], uy = ruy[e] / rho[e];
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) void k_com_x2(F2XArgs x, const double2* tw)
         const double p = x.p0 * rho[e];
         f0[e] = ((ruy[e] * uy + p) + b2 / 2.0) - by[e] * by[e];
         // The reference sets d_Bx_dy and d_By_dx from the already normalised
-        // d_ux_dx and d_uy_dy, divided by Nx*Ny once more (derive2com.f95:172-175).
+        // d_ux_dx and d_uy_dy, divided by Nx*Ny once more (derive2com.f95:186-187).
         const double dbxdy = dudx[e] * x.norm, dbydx = dvdy[e] * x.norm;
         const double curl = dbydx - dbxdy;
         const double ex = (((en[e] + p) + b2 / 2.0) * ux - (ux * bx[e]) * bx[e]) - (uy * bx[e]) * by[e];
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(CShape<NY>::NT) void k_com_yfwd(F2YArgs a, const do
     F2Y_PROLOGUE(NY)
     double2 v[E], acc[E], acc2[E];
     auto msk = [&](int e, double2 z) { return keep[e] ? z : make_double2(0.0, 0.0); };
-    // density: -(i kx rho ux + i ky rho uy), masked (derive2com.f95:252, 328, 408)
+    // density: -(i kx rho ux + i ky rho uy), masked (derive2com.f95:267, 339, 402)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const double2 s1 = msk(e, f2_ik(kxv, a.st[1][q[e]])), s2 = msk(e, f2_ik(kyv[e], a.st[2][q[e]]));
