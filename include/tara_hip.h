@@ -274,6 +274,17 @@ int tara_mhd_destroy(tara_mhd* ctx);
 int tara_mhd_set_timing(tara_mhd* ctx, int on);
 int tara_mhd_kernel_stats(tara_mhd* ctx, int nclass, long* counts, double* ms);
 
+/* ------------------------------------------------- matrix-core GEMM (FNO)
+ * No counterpart in the reference.  C[z](m, n) = (or +=) alpha * sum_k
+ * A[z](m, k) B[z](k, n) for z = (z1, z2), z1 < nb1, z2 < nb2, on device
+ * pointers; element (m, k) of A at a + m*sam + k*sak + z1*sa1 + z2*sa2 (elements),
+ * likewise B and C.  dtype 0: A, B bfloat16 (MFMA 32x32x8, FP32 accumulation);
+ * 1: float (MFMA 32x32x2).  epi 0: store float, 1: add into float, 2: store
+ * bfloat16 (round to nearest even).  Synchronous. */
+int tara_gemm(int dtype, int epi, long m, long n, long k, const void* a, long sam, long sak, const void* b, long sbk,
+              long sbn, void* c, long scm, long scn, int nb1, long sa1, long sb1, long sc1, int nb2, long sa2,
+              long sb2, long sc2, float alpha);
+
 /* ------------------------------------------------------ snapshots and data
  * Host I/O, no GPU needed.
  * Checkpoints (FP64): a 64-byte header ("TARACKPT", version 1, kind 1, nx, ny,

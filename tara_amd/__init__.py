@@ -90,6 +90,10 @@ def lib():
         L.tara_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
         L.tara_csh_tune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]
         L.tara_csh_set_xshape.argtypes = [C.c_void_p, C.c_int]
+        L.tara_gemm.argtypes = ([C.c_int, C.c_int, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,
+                                 C.c_void_p, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,
+                                 C.c_int, C.c_long, C.c_long, C.c_long, C.c_int, C.c_long, C.c_long, C.c_long,
+                                 C.c_float])
         L.tara_ens_create.argtypes = [C.c_int, C.POINTER(C.c_int), _dp, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
         L.tara_ens_upload.argtypes = [C.c_void_p, C.c_int, _dp]
         L.tara_ens_download.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -304,6 +308,17 @@ class CshSolver:
             self.close()
         except Exception:
             pass
+
+
+GEMM_BF16, GEMM_F32 = 0, 1
+GEMM_STORE_F32, GEMM_ACCUM_F32, GEMM_STORE_BF16 = 0, 1, 2
+
+
+def gemm(dtype, epi, m, n, k, a, sam, sak, b, sbk, sbn, c, scm, scn,
+         nb1=1, sa1=0, sb1=0, sc1=0, nb2=1, sa2=0, sb2=0, sc2=0, alpha=1.0):
+    """tara_gemm on device pointers (ints), strides in elements."""
+    _check(lib().tara_gemm(dtype, epi, m, n, k, a, sam, sak, b, sbk, sbn, c, scm, scn,
+                           nb1, sa1, sb1, sc1, nb2, sa2, sb2, sc2, alpha))
 
 
 def philox4x32_10(ctr, key):
