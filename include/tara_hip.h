@@ -129,6 +129,15 @@ int tara_ens_sync(tara_ens* ctx);
 int tara_ens_use_graph(tara_ens* ctx, int on);
 int tara_ens_tune(tara_ens* ctx, int reps);
 int tara_ens_destroy(tara_ens* ctx);
+/* Device-side data paths of one run, ordered on the ensemble's stream:
+ * ic_random sets the random-phase vorticity of tara_csh_ic_random (synchronous);
+ * set_omega / get_omega move real-space vorticity (device float, n*n, x fastest)
+ * into / out of the FP64 state through the library's FFTs (asynchronous;
+ * set_omega zeroes the run's AB2 history; get_omega's pointer must stay valid
+ * until tara_ens_sync). */
+int tara_ens_ic_random(tara_ens* ctx, int run, unsigned seed, double k0, double energy);
+int tara_ens_set_omega(tara_ens* ctx, int run, const float* omega);
+int tara_ens_get_omega(tara_ens* ctx, int run, float* omega);
 
 /* ------------------------------------- 2D screened vorticity (css family)
  * Replaces src/derive2css.f95:6 derive2css(Nx,Ny,Nh,alpha,nu,time,omegak,
@@ -284,6 +293,42 @@ int tara_mhd_kernel_stats(tara_mhd* ctx, int nclass, long* counts, double* ms);
 int tara_gemm(int dtype, int epi, long m, long n, long k, const void* a, long sam, long sak, const void* b, long sbk,
               long sbn, void* c, long scm, long scn, int nb1, long sa1, long sb1, long sc1, int nb2, long sa2,
               long sb2, long sc2, float alpha);
+
+/* ------------------------------------------- Fourier-neural-operator surrogate
+ * No counterpart in the reference.  Maps real-space vorticity omega(t) (device
+ * float [batch][n][n], x fastest) to omega(t + T): lift, `layers` Fourier
+ * layers of width `width` keeping kx < mx and -my/2 <= ky < my/2, GELU,
+ * projection through 2*width hidden channels, per-sample amplitude scaling and
+ * a residual connection (parameter layout in tara_amd/csrc/fno.hip).
+ * precision 0: bfloat16 operands with FP32 accumulation on the matrix cores;
+ * 1: FP32 operands throughout.  Training minimises the batch mean of the
+ * relative L2 error |out - target| / |target| with Adam (0.9, 0.999, 1e-8).
+ * forward, grads, train_step and rel_err are synchronous; get/set_params and
+ * grads' grad_host use host float arrays of tara_fno_num_params entries.
+ * rel_err: err[b] = |a_b - ref_b| / |ref_b| over device arrays. */
+typedef struct tara_fno tara_fno;
+int tara_fno_create(int n, int width, int mx, int my, int layers, int max_batch, int precision, unsigned seed,
+                    tara_fno** out);
+int tara_fno_num_params(tara_fno* f, long* n);
+int tara_fno_get_params(tara_fno* f, float* host);
+int tara_fno_set_params(tara_fno* f, const float* host);
+int tara_fno_forward(tara_fno* f, int batch, const float* omega, float* out);
+int tara_fno_grads(tara_fno* f, int batch, const float* omega, const float* target, float* loss, float* grad_host);
+int tara_fno_train_step(tara_fno* f, int batch, const float* omega, const float* target, float lr, float* loss);
+int tara_fno_rel_err(tara_fno* f, int batch, const float* a, const float* ref, float* err_host);
+/* Surrogate rollout with the FP64 solver as checker.  traj: device float
+ * [nsteps+1][batch][n][n], traj[0] set by the caller; step t+1 = network(step t).
+ * Every `every`-th step t+1 = (j+1)*every is re-checked: checker ensemble j
+ * (runs 0..batch-1 of grid n, the solver's viscosity, dt and scheme) advances
+ * traj[t] by solver_steps steps on its own streams, concurrently with the
+ * rollout; nchk = nsteps / every checkers are needed.  When a check's relative
+ * L2 error exceeds tol, the solver's field replaces that sample's step (j+1)*every
+ * and the rollout resumes from there.  check_err (host, [nchk][batch]) receives
+ * each check's error, fallbacks the number of take-overs.  use_graph replays a
+ * captured network step.  Synchronous. */
+int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps, int every, int nchk, tara_ens* const* checkers,
+                     int solver_steps, float tol, int use_graph, float* check_err, int* fallbacks);
+int tara_fno_destroy(tara_fno* f);
 
 /* ------------------------------------------------------ snapshots and data
  * Host I/O, no GPU needed.

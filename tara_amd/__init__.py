@@ -102,6 +102,22 @@ def lib():
             getattr(L, n).argtypes = [C.c_void_p]
         L.tara_ens_use_graph.argtypes = [C.c_void_p, C.c_int]
         L.tara_ens_tune.argtypes = [C.c_void_p, C.c_int]
+        L.tara_ens_ic_random.argtypes = [C.c_void_p, C.c_int, C.c_uint, C.c_double, C.c_double]
+        L.tara_ens_set_omega.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tara_ens_get_omega.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        _fp = C.POINTER(C.c_float)
+        L.tara_fno_create.argtypes = [C.c_int] * 7 + [C.c_uint, C.POINTER(C.c_void_p)]
+        L.tara_fno_num_params.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
+        L.tara_fno_get_params.argtypes = [C.c_void_p, _fp]
+        L.tara_fno_set_params.argtypes = [C.c_void_p, _fp]
+        L.tara_fno_forward.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.tara_fno_grads.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _fp, _fp]
+        L.tara_fno_train_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, _fp]
+        L.tara_fno_rel_err.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _fp]
+        L.tara_fno_destroy.argtypes = [C.c_void_p]
+        L.tara_fno_rollout.argtypes = ([C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(C.c_void_p), C.c_int, C.c_float, C.c_int, _fp,
+                                        C.POINTER(C.c_int)])
         P3, P6 = C.c_void_p * 3, C.c_void_p * 6
         L.tara_derive2coh.argtypes = [C.c_int] * 3 + [C.POINTER(CohParams), P3, P3]
         L.tara_ab2coh.argtypes = [C.c_int] * 3 + [C.c_double, P3, P3, P3, P3]
@@ -321,6 +337,86 @@ def gemm(dtype, epi, m, n, k, a, sam, sak, b, sbk, sbn, c, scm, scn,
                            nb1, sa1, sb1, sc1, nb2, sa2, sb2, sc2, alpha))
 
 
+def _dptr(x):
+    """Device address of a torch device tensor, or an int taken as one."""
+    return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
+
+
+FNO_BF16, FNO_FP32 = 0, 1
+
+
+class FNO:
+    """Fourier-neural-operator surrogate on the matrix cores (tara_fno_*):
+    omega(t) -> omega(t + T) for batches of n x n real-space vorticity fields in
+    device float32 memory (torch tensors or device addresses)."""
+
+    def __init__(self, n, width=64, mx=16, my=32, layers=4, max_batch=16, precision=FNO_BF16, seed=0):
+        self.n, self.width, self.mx, self.my, self.layers = n, width, mx, my, layers
+        self.max_batch = max_batch
+        h = C.c_void_p()
+        _check(lib().tara_fno_create(n, width, mx, my, layers, max_batch, precision, seed, C.byref(h)))
+        self._h = h
+        k = C.c_long()
+        _check(lib().tara_fno_num_params(h, C.byref(k)))
+        self.nparams = k.value
+
+    def get_params(self):
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().tara_fno_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        _check(lib().tara_fno_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, batch, omega, out):
+        _check(lib().tara_fno_forward(self._h, batch, _dptr(omega), _dptr(out)))
+
+    def grads(self, batch, omega, target):
+        """(loss, gradient as float32 numpy) of the batch-mean relative L2 error."""
+        loss = C.c_float()
+        g = np.zeros(self.nparams, np.float32)
+        _check(lib().tara_fno_grads(self._h, batch, _dptr(omega), _dptr(target), C.byref(loss),
+                                    g.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, g
+
+    def train_step(self, batch, omega, target, lr):
+        loss = C.c_float()
+        _check(lib().tara_fno_train_step(self._h, batch, _dptr(omega), _dptr(target), lr, C.byref(loss)))
+        return loss.value
+
+    def rel_err(self, batch, a, ref):
+        out = np.zeros(batch, np.float32)
+        _check(lib().tara_fno_rel_err(self._h, batch, _dptr(a), _dptr(ref),
+                                      out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def rollout(self, batch, traj, nsteps, every=10, checkers=(), solver_steps=100, tol=0.05, graph=True):
+        """Surrogate rollout of traj[0] (device float32 [nsteps+1][batch][n][n])
+        with FP64 checks every `every` steps by the Ensemble objects in
+        `checkers` (nsteps // every of them, each with `batch` runs of this grid).
+        Returns (check errors [nchk][batch], number of solver take-overs)."""
+        nchk = len(checkers)
+        hs = (C.c_void_p * max(nchk, 1))(*[c._h for c in checkers])
+        err = np.zeros(max(nchk * batch, 1), np.float32)
+        nf = C.c_int()
+        _check(lib().tara_fno_rollout(self._h, batch, _dptr(traj), nsteps, every, nchk, hs, solver_steps, tol,
+                                      1 if graph else 0, err.ctypes.data_as(C.POINTER(C.c_float)), C.byref(nf)))
+        return err[:nchk * batch].reshape(nchk, batch), nf.value
+
+    def close(self):
+        if self._h:
+            lib().tara_fno_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def philox4x32_10(ctr, key):
     """Host evaluation of the generator behind CshSolver.ic_random."""
     c = (C.c_uint * 4)(*ctr)
@@ -364,6 +460,19 @@ class Ensemble:
 
     def use_graph(self, on):
         _check(lib().tara_ens_use_graph(self._h, 1 if on else 0))
+
+    def ic_random(self, run, seed, k0=4.0, energy=1.0):
+        """Random-phase vorticity of CshSolver.ic_random for one run."""
+        _check(lib().tara_ens_ic_random(self._h, run, seed, k0, energy))
+
+    def set_omega(self, run, omega):
+        """Real-space vorticity (device float32, n x n, x fastest) into one run."""
+        _check(lib().tara_ens_set_omega(self._h, run, _dptr(omega)))
+
+    def get_omega(self, run, out):
+        """One run's real-space vorticity into device float32 memory (asynchronous
+        until sync())."""
+        _check(lib().tara_ens_get_omega(self._h, run, _dptr(out)))
 
     def tune(self, reps=3):
         _check(lib().tara_ens_tune(self._h, reps))

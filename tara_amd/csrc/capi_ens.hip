@@ -11,6 +11,8 @@
 
 #include "common.h"
 #include "csh2d.h"
+#include "ens.h"
+#include "fft_api.h"
 #include "layout.h"
 
 using namespace tara;
@@ -22,6 +24,7 @@ struct Bucket {
     std::vector<int> runs;  // run ids, in bucket order
     CshPlan plan;           // batch = runs.size()
     DevBuf w, aux, next, stage, nus;
+    DevBuf spec, real;  // real <-> spectral workspace of one run (allocated on first use)
     CshPhys ph;
     hipStream_t st = nullptr;
     hipEvent_t done = nullptr;
@@ -98,6 +101,52 @@ int run_slot(tara_ens* e, int run, Bucket** b, int* slot)
     TARA_CHECK(e && run >= 0 && run < (int)e->where.size(), TARA_ERR_ARG, "run %d out of range", run);
     *b = e->b[e->where[run].first].get();
     *slot = e->where[run].second;
+    return TARA_OK;
+}
+
+// w[i*ny + j] = s[j*ld + i]: padded kx-fastest spectrum -> ky-major state
+__global__ void k_spec_to_state(const double2* s, int ld, int nh, int ny, double2* w)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)nh * ny;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q % ny), i = (int)(q / ny);
+        w[q] = s[(size_t)j * ld + i];
+    }
+}
+
+// s[j*ld + i] = w[i*ny + j] for i < nh, 0 in the padding columns
+__global__ void k_state_to_spec(const double2* w, int nh, int ny, int ld, double2* s)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)ld * ny;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % ld), j = (int)(q / ld);
+        s[q] = i < nh ? w[(size_t)i * ny + j] : make_double2(0.0, 0.0);
+    }
+}
+
+__global__ void k_f32_to_f64(const float* a, double* b, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        b[q] = (double)a[q];
+}
+
+__global__ void k_f64_to_f32(const double* a, float* b, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        b[q] = (float)a[q];
+}
+
+unsigned ens_grid(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b ? b : 1));
+}
+
+int bucket_ws(Bucket& b)
+{
+    int rc;
+    if (!b.spec.p && (rc = b.spec.alloc((size_t)spec_ld(b.n) * b.n * sizeof(double2)))) return rc;
+    if (!b.real.p && (rc = b.real.alloc((size_t)b.n * b.n * sizeof(double)))) return rc;
     return TARA_OK;
 }
 
@@ -178,6 +227,57 @@ extern "C" int tara_ens_download(tara_ens* e, int run, double* omegak)
     if ((rc = from_kymajor(b->w.as<double2>() + slot * p.bstride, b->stage.as<double2>(), p.nh, p.ny, 1, nullptr)))
         return rc;
     TARA_HIP(hipMemcpy(omegak, b->stage.p, b->stage.bytes, hipMemcpyDeviceToHost));
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_ic_random(tara_ens* e, int run, unsigned seed, double k0, double energy)
+{
+    Bucket* b = nullptr;
+    int slot = 0, rc;
+    if ((rc = run_slot(e, run, &b, &slot))) return rc;
+    const CshPlan& p = b->plan;
+    if ((rc = csh_random_phase(b->w.as<double2>() + slot * p.bstride, b->n, b->n, seed, k0, energy, e->st)))
+        return rc;
+    TARA_HIP(hipMemsetAsync(b->aux.as<double2>() + slot * p.bstride, 0, p.bstride * sizeof(double2), e->st));
+    TARA_HIP(hipStreamSynchronize(e->st));
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_set_omega(tara_ens* e, int run, const float* omega)
+{
+    TARA_CHECK(omega, TARA_ERR_ARG, "null array");
+    Bucket* b = nullptr;
+    int slot = 0, rc;
+    if ((rc = run_slot(e, run, &b, &slot)) || (rc = bucket_ws(*b))) return rc;
+    const CshPlan& p = b->plan;
+    const int n = b->n, ld = spec_ld(n);
+    const size_t np = (size_t)n * n;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(ens_grid(np)), dim3(256), 0, e->st, omega, b->real.as<double>(), np);
+    if ((rc = row_r2c(b->real.as<double>(), b->spec.as<double2>(), n, ld, n, e->st))) return rc;
+    if ((rc = col_fft(b->spec.as<double2>(), n, ld, ld, (long)ld * n, 1, -1, e->st))) return rc;
+    hipLaunchKernelGGL(k_spec_to_state, dim3(ens_grid((size_t)p.nh * n)), dim3(256), 0, e->st,
+                       b->spec.as<double2>(), ld, p.nh, n, b->w.as<double2>() + slot * p.bstride);
+    TARA_HIP(hipGetLastError());
+    TARA_HIP(hipMemsetAsync(b->aux.as<double2>() + slot * p.bstride, 0, p.bstride * sizeof(double2), e->st));
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_get_omega(tara_ens* e, int run, float* omega)
+{
+    TARA_CHECK(omega, TARA_ERR_ARG, "null array");
+    Bucket* b = nullptr;
+    int slot = 0, rc;
+    if ((rc = run_slot(e, run, &b, &slot)) || (rc = bucket_ws(*b))) return rc;
+    const CshPlan& p = b->plan;
+    const int n = b->n, ld = spec_ld(n);
+    const size_t np = (size_t)n * n;
+    hipLaunchKernelGGL(k_state_to_spec, dim3(ens_grid((size_t)ld * n)), dim3(256), 0, e->st,
+                       b->w.as<double2>() + slot * p.bstride, p.nh, n, ld, b->spec.as<double2>());
+    TARA_HIP(hipGetLastError());
+    if ((rc = col_fft(b->spec.as<double2>(), n, ld, ld, (long)ld * n, 1, +1, e->st))) return rc;
+    if ((rc = row_c2r(b->spec.as<double2>(), b->real.as<double>(), n, ld, n, 1.0 / (double)np, e->st))) return rc;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(ens_grid(np)), dim3(256), 0, e->st, b->real.as<double>(), omega, np);
+    TARA_HIP(hipGetLastError());
     return TARA_OK;
 }
 
@@ -266,3 +366,12 @@ extern "C" int tara_ens_destroy(tara_ens* e)
     delete e;
     return TARA_OK;
 }
+
+namespace tara {
+hipStream_t ens_stream(tara_ens* e) { return e ? e->st : nullptr; }
+int ens_grid_of(tara_ens* e, int run)
+{
+    if (!e || run < 0 || run >= (int)e->where.size()) return 0;
+    return e->b[e->where[run].first]->n;
+}
+}  // namespace tara

@@ -70,6 +70,9 @@ int csh_default_xshape(int nx);
 int csh_num_xshapes();
 int csh_xshape_ok(const CshPlan& p, int c);
 int csh_xpass(const CshPlan& p, hipStream_t st);
+// Random-phase vorticity (ky-major state of an nx x ny run): amplitude
+// sqrt(k E(k)/pi), E(k) = k^4 exp(-(k/k0)^2), scaled to kinetic energy `energy`.
+int csh_random_phase(double2* w, int nx, int ny, unsigned seed, double k0, double energy, hipStream_t st);
 // Access to a tara_csh context's device state (capi_csh.hip): the ky-major
 // omega_hat, the plan and the stream; and zeroing of its AB2 history.
 int csh_ctx_state(tara_csh* c, double2** w, const CshPlan** p, hipStream_t* st);

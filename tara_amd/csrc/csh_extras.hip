@@ -214,6 +214,20 @@ extern "C" int tara_csh_ic_shear(tara_csh* c, double lx_in, double ly_in, double
     return csh_ctx_zero_history(c);
 }
 
+namespace tara {
+int csh_random_phase(double2* w, int nx, int ny, unsigned seed, double k0, double energy, hipStream_t st)
+{
+    TARA_CHECK(k0 > 0 && energy >= 0, TARA_ERR_ARG, "ic: k0 must be positive, energy non-negative");
+    const double e0 = random_phase_energy(nx, ny, k0);
+    const double scale = e0 > 0 ? std::sqrt(energy / e0) : 0.0;
+    const int nh = nx / 2 + 1;
+    hipLaunchKernelGGL(k_random_phase, dim3(grid_for((size_t)nh * ny)), dim3(256), 0, st, w, nh, ny, seed, k0,
+                       scale);
+    TARA_HIP(hipGetLastError());
+    return TARA_OK;
+}
+}  // namespace tara
+
 extern "C" int tara_csh_ic_random(tara_csh* c, unsigned seed, double k0, double energy)
 {
     double2* w = nullptr;
@@ -221,12 +235,7 @@ extern "C" int tara_csh_ic_random(tara_csh* c, unsigned seed, double k0, double 
     hipStream_t st = nullptr;
     int rc = csh_ctx_state(c, &w, &p, &st);
     if (rc) return rc;
-    TARA_CHECK(k0 > 0 && energy >= 0, TARA_ERR_ARG, "ic: k0 must be positive, energy non-negative");
-    const double e0 = random_phase_energy(p->nx, p->ny, k0);
-    const double scale = e0 > 0 ? std::sqrt(energy / e0) : 0.0;
-    hipLaunchKernelGGL(k_random_phase, dim3(grid_for((size_t)p->nh * p->ny)), dim3(256), 0, st, w, p->nh, p->ny,
-                       seed, k0, scale);
-    TARA_HIP(hipGetLastError());
+    if ((rc = csh_random_phase(w, p->nx, p->ny, seed, k0, energy, st))) return rc;
     TARA_HIP(hipStreamSynchronize(st));
     return csh_ctx_zero_history(c);
 }

@@ -49,17 +49,20 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     v16f acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    // Lanes walk the operand's unit-stride dimension when it has one, so that
+    // each fetch instruction reads contiguous memory.
+    const bool a_kfast = g.sak == 1, b_kfast = g.sbk == 1;
     for (long k0 = 0; k0 < g.k; k0 += GBK) {
         // stage A (64 x 32) and B (32 x 64): 8 elements per thread each
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int e = tid + r * 256;
-            const int am = e / GBK, ak = e % GBK;  // k fastest across lanes
+            const int am = a_kfast ? e / GBK : e % GBM, ak = a_kfast ? e % GBK : e / GBM;
             const long gm = m0 + am, gk = k0 + ak;
             T va = T(0);
             if (gm < g.m && gk < g.k) va = A[gm * g.sam + gk * g.sak];
             As[am][ak] = va;
-            const int bn = e / GBK, bk = e % GBK;
+            const int bn = b_kfast ? e / GBK : e % GBN, bk = b_kfast ? e % GBK : e / GBN;
             const long gn = n0 + bn, gk2 = k0 + bk;
             T vb = T(0);
             if (gn < g.n && gk2 < g.k) vb = B[gk2 * g.sbk + gn * g.sbn];
