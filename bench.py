@@ -159,6 +159,90 @@ def bench_ensemble(steps, warmup, nruns=512, seed=7):
             "step_model_bytes": 60 * S_all, "step_model_GBps": round(60 * S_all / t_ens / 1e9, 1)}
 
 
+def fno_forward_flops(n, c, mx, my, layers):
+    """Algorithmic FLOPs of one network step for one sample (the products of
+    tara_amd/csrc/fno.hip's forward: truncated DFTs, per-mode contraction,
+    inverse DFTs, channel mixing, projection)."""
+    np_ = n * n
+    layer = (2 * 2 * mx * n * np_ // n * c        # x transform: (2MX x n) (n x n) per channel
+             + 2 * 2 * my * 2 * n * mx * c        # y transform
+             + 2 * (2 * c) * (2 * c) * mx * my    # per-mode contraction
+             + 2 * 2 * n * 2 * my * mx * c        # inverse y
+             + 2 * c * n * 2 * mx * n             # inverse x
+             + 2 * c * c * np_)                   # channel mixing
+    return layers * layer + 2 * 2 * c * c * np_ + 2 * 2 * c * np_
+
+
+def bench_fno(epochs, nseeds=400, nheld=8):
+    """Secondary line: the FNO surrogate of 2D hydro at 256^2, nu 1e-3, one
+    network step = 100 solver steps (dt 5e-4).  Trains on nseeds x 50 pairs
+    from the FP64 ensemble (k0 uniform in [3, 8]), then rolls out 50 steps from
+    nheld held-out seeds, re-checked by the FP64 solver every 10th step with
+    take-over above 5 % relative L2 error; errors are measured against FP64
+    reference trajectories, speed against the single-run FP64 solver."""
+    import numpy as np
+    import torch
+
+    import tara_amd as T
+    from tara_amd import surrogate as S
+
+    n, nu, dt, steps, width, mx, my, layers, batch = 256, 1e-3, 5e-4, 100, 64, 16, 32, 4, 16
+    t0 = time.perf_counter()
+    snaps, _ = S.generate(n, nu, dt, steps, range(1, nseeds + 1), 50)
+    held, _ = S.generate(n, nu, dt, steps, range(100001, 100001 + nheld), 50, k0_seed=99)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    net = T.FNO(n, width, mx, my, layers, max_batch=batch, seed=3)
+    xb = snaps[0, :batch].contiguous()
+    net.train_step(batch, xb, snaps[1, :batch].contiguous(), 0.0)  # warm
+    t0 = time.perf_counter()
+    hist = S.train(net, snaps, epochs, batch, lr=1e-3, lr_min=5e-5)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    npairs = 50 * nseeds
+    e_net, e_id = S.one_step_error(net, held)
+    out = torch.empty((batch, n, n), device="cuda")
+    net.forward(batch, snaps[0, :batch], out)
+    reps = 20
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(batch, snaps[0, :batch], out)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / reps
+    chk = S.rollout(net, held, nu, dt, steps, every=10, tol=0.05, reps=2)
+    free = S.rollout(net, held, nu, dt, steps, every=51, tol=0.05, reps=2)
+    nsteps = held.shape[0] - 1
+    sim_time = nsteps * steps * dt
+    solver_ms = S.solver_time(n, nu, dt, nsteps * steps)
+    med = lambda v: float(np.median(v))
+    flops = fno_forward_flops(n, width, mx, my, layers)
+    return {
+        "workload": f"FNO surrogate, 2D hydro {n}^2, nu {nu}, step = {steps} solver steps (dt {dt}); "
+                    f"{layers} Fourier layers, width {width}, modes kx<{mx} x |ky|<={my // 2}, bf16 MFMA",
+        "unit": "sim-time/s",
+        "solver_value": sim_time / (solver_ms * 1e-3),
+        "checked_value": sim_time / (med(chk["ms"]) * 1e-3),
+        "unchecked_value": sim_time / (med(free["ms"]) * 1e-3),
+        "speedup_checked": solver_ms / med(chk["ms"]),
+        "speedup_unchecked": solver_ms / med(free["ms"]),
+        "rollout_steps": nsteps, "check_every": 10, "tol": 0.05,
+        "checked_final_rel_l2": [round(v, 4) for v in chk["final"]],
+        "checked_fallbacks": chk["fallbacks"],
+        "unchecked_final_rel_l2": [round(v, 4) for v in free["final"]],
+        "one_step_rel_l2": e_net, "persistence_one_step_rel_l2": e_id,
+        "target_met": bool(med(chk["final"]) <= 0.05 and solver_ms / med(chk["ms"]) >= 20),
+        "train_pairs": npairs, "epochs": epochs, "epoch_loss": [round(v, 4) for v in hist],
+        "train_s": t_train, "train_pairs_per_s": epochs * (npairs // batch) * batch / t_train,
+        "datagen_s": t_gen,
+        "forward_ms_batch16": fwd_ms,
+        "forward_tflops_batch16": flops * batch / (fwd_ms * 1e-3) / 1e12,
+        "forward_gflop_per_sample": flops / 1e9,
+        "solver_ms_per_rollout": solver_ms,
+        "checked_ms_per_rollout": med(chk["ms"]), "unchecked_ms_per_rollout": med(free["ms"]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +252,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-3d", action="store_true", help="skip the secondary 3D MHD measurement")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the secondary ensemble measurement")
+    ap.add_argument("--no-fno", action="store_true", help="skip the secondary FNO surrogate measurement")
+    ap.add_argument("--fno-epochs", type=int, default=3, help="training epochs of the FNO leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -178,6 +264,10 @@ def main():
 
     # CPU baseline first, in a child process, before this process touches the GPU.
     cpu = None if a.no_cpu_baseline else cpu_baseline(n)
+    if not a.no_fno:
+        # torch (device memory of the surrogate leg) is loaded before the HIP
+        # runtime starts in this process: loaded after, it finds no device.
+        import torch  # noqa: F401
 
     import tara_amd as T
     from tara_amd.ic import random_phase_vorticity
@@ -218,6 +308,7 @@ def main():
                           "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}
     mhd = None if a.no_3d else bench_mhd(a.steps, a.warmup)
     ens = None if a.no_ensemble else bench_ensemble(min(a.steps, 20), a.warmup)
+    fno = None if a.no_fno else bench_fno(a.fno_epochs)
     out = {
         "metric": METRIC,
         "value": n * n / (ms_per_step * 1e-3),
@@ -246,6 +337,7 @@ def main():
         "cpu_baseline": cpu,
         "secondary_3d_mhd": mhd,
         "secondary_ensemble": ens,
+        "secondary_fno": fno,
     }
     print(json.dumps(out))
 

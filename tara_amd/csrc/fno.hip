@@ -78,17 +78,6 @@ __global__ void k_to_bf16(const float* x, uint16_t* y, size_t n)
     FNO_LOOP(n) y[i] = fno_f2bf(x[i]);
 }
 
-// sig[b] = rms of sample b (one workgroup per sample)
-__global__ __launch_bounds__(256) void k_rms(const float* om, float* sig, size_t np)
-{
-    __shared__ float sh[256];
-    const float* o = om + blockIdx.x * np;
-    float a = 0.f;
-    for (size_t p = threadIdx.x; p < np; p += 256) a += o[p] * o[p];
-    const float s = block_sum(a, sh);
-    if (threadIdx.x == 0) sig[blockIdx.x] = sqrtf(s / (float)np) + 1e-20f;
-}
-
 // v0[b][c][p] = Wl[c] om[b][p] / sig[b] + bl[c]  (null outputs are skipped)
 __global__ void k_lift(const float* om, const float* sig, const float* wl, const float* bl, float* v, uint16_t* vb,
                        int nb, int c, size_t np)
@@ -194,60 +183,57 @@ __global__ void k_proj2_back(const float* dproj, const float* w2, const float* h
     }
 }
 
-// Workgroup j < hdim: dw2[j] += sum_{b,p} h[b][j][p] dproj[b][p]; workgroup hdim: db2 += sum dproj.
-__global__ __launch_bounds__(256) void k_proj2_wgrad(const float* h, const float* dproj, float* dw2, float* db2,
-                                                     int nb, int hdim, size_t np)
-{
-    __shared__ float sh[256];
-    const int j = blockIdx.x;
-    float acc = 0.f;
-    for (size_t q = threadIdx.x; q < (size_t)nb * np; q += 256) {
-        const size_t b = q / np, p = q % np;
-        acc += (j < hdim ? h[(b * hdim + j) * np + p] : 1.f) * dproj[q];
-    }
-    const float s = block_sum(acc, sh);
-    if (threadIdx.x == 0) {
-        if (j < hdim)
-            dw2[j] += s;
-        else
-            db2[0] += s;
-    }
-}
-
-// dbias[c] += sum_{b,p} g[b][c][p]  (one workgroup per channel)
-__global__ __launch_bounds__(256) void k_bias_grad(const float* g, float* db, int nb, int c, size_t np)
-{
-    __shared__ float sh[256];
-    const int ch = blockIdx.x;
-    float acc = 0.f;
-    for (size_t q = threadIdx.x; q < (size_t)nb * np; q += 256) acc += g[((q / np) * c + ch) * np + q % np];
-    const float s = block_sum(acc, sh);
-    if (threadIdx.x == 0) db[ch] += s;
-}
-
 __global__ void k_act_back(float* dv, const float* pre, size_t n)
 {
     FNO_LOOP(n) dv[i] *= gelu_grad(pre[i]);
 }
 
-// Lift: dWl[c] += sum dv0 om / sig, dbl[c] += sum dv0  (one workgroup per channel)
-__global__ __launch_bounds__(256) void k_lift_grad(const float* dv, const float* om, const float* sig, float* dwl,
-                                                   float* dbl, int nb, int c, size_t np)
+// Reductions over every pixel of every sample run as RED_SPLIT workgroups per
+// output, each over one contiguous slice, and a second pass over the partials.
+constexpr int RED_SPLIT = 64;
+
+// part[s][ch] = sum over slice s of q = b np + p < nb np of g[b][ch][p] w[q] / sig[b]
+// (w, sig optional); grid (c, RED_SPLIT).
+__global__ __launch_bounds__(256) void k_chan_dot(const float* g, const float* w, int c, size_t np, int nb,
+                                                  const float* sig, float* part)
 {
     __shared__ float sh[256];
-    const int ch = blockIdx.x;
-    float a0 = 0.f, a1 = 0.f;
-    for (size_t q = threadIdx.x; q < (size_t)nb * np; q += 256) {
-        const size_t b = q / np, p = q % np;
-        const float g = dv[(b * c + ch) * np + p];
-        a0 += g * (om[q] / sig[b]);
-        a1 += g;
+    const int ch = blockIdx.x, sl = blockIdx.y;
+    const size_t total = (size_t)nb * np, chunk = (total + RED_SPLIT - 1) / RED_SPLIT;
+    const size_t q0 = sl * chunk, q1 = q0 + chunk < total ? q0 + chunk : total;
+    float acc = 0.f;
+    for (size_t q = q0 + threadIdx.x; q < q1; q += 256) {
+        const size_t b = q / np, p = q - b * np;
+        float v = g[(b * c + ch) * np + p];
+        if (w) v *= w[q];
+        if (sig) v /= sig[b];
+        acc += v;
     }
-    const float s0 = block_sum(a0, sh), s1 = block_sum(a1, sh);
-    if (threadIdx.x == 0) {
-        dwl[ch] += s0;
-        dbl[ch] += s1;
-    }
+    const float r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(size_t)sl * c + ch] = r;
+}
+
+// part[b][s] = sum of om[b][p]^2 over slice s of the pixels; grid (nb, RED_SPLIT).
+__global__ __launch_bounds__(256) void k_rms_part(const float* om, size_t np, float* part)
+{
+    __shared__ float sh[256];
+    const int b = blockIdx.x, sl = blockIdx.y;
+    const size_t chunk = (np + RED_SPLIT - 1) / RED_SPLIT, p0 = sl * chunk, p1 = p0 + chunk < np ? p0 + chunk : np;
+    const float* o = om + (size_t)b * np;
+    float acc = 0.f;
+    for (size_t p = p0 + threadIdx.x; p < p1; p += 256) acc += o[p] * o[p];
+    const float r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(size_t)b * RED_SPLIT + sl] = r;
+}
+
+// sig[b] = sqrt(sum_s part[b][s] / np)
+__global__ void k_rms_final(const float* part, int nb, size_t np, float* sig)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    float s = 0.f;
+    for (int k = 0; k < RED_SPLIT; ++k) s += part[(size_t)b * RED_SPLIT + k];
+    sig[b] = sqrtf(s / (float)np) + 1e-20f;
 }
 
 // out[i] += sum_{p < nparts} part[p size + i]
@@ -325,10 +311,10 @@ struct tara_fno {
     size_t r_sz = 0;  // spectral weights per layer
     DevBuf par, grad, m1, m2, wb;  // parameters, gradients, Adam moments, bfloat16 copy of the parameters
     DevBuf rs, rsb;                // stacked per-mode matrices of every layer (float, bfloat16)
-    DevBuf fxc, fxs, fyre, fyim, gc, gs, hx;  // DFT matrices (float)
-    DevBuf fxcb, fxsb, fyreb, fyimb, gcb, gsb;  // the same in bfloat16
+    DevBuf fxt, fy2, g2, hx;  // truncated DFT matrices (float)
+    DevBuf fxtb, fy2b, g2b;   // the same in bfloat16
     // activations, sized for maxb samples
-    DevBuf sig, v, vb, pre, x, xb, t1, y, zf, kv, hpre, hb, out, dproj, loss;
+    DevBuf sig, v, vb, pre, x, xb, t1, y, yb, zf, kv, hpre, hb, out, dproj, loss;
     DevBuf dva, dvb, dz, dy, dx, dt1, drs, dhpre, part;
     int step = 0;
     hipStream_t st = nullptr;
@@ -401,94 +387,114 @@ int wgrad(tara_fno* f, long m, long n, long k, const float* A, long sam, long sa
     return TARA_OK;
 }
 
+// sig[b] = rms(omega_b): partial sums of squares over RED_SPLIT slices, then a
+// second pass.
+int rms(tara_fno* f, int nb, const float* om)
+{
+    hipLaunchKernelGGL(k_rms_part, dim3(nb, RED_SPLIT), dim3(256), 0, f->st, om, f->np, P<float>(f->part));
+    hipLaunchKernelGGL(k_rms_final, dim3((nb + 255) / 256), dim3(256), 0, f->st, P<float>(f->part), nb, f->np,
+                       P<float>(f->sig));
+    TARA_HIP(hipGetLastError());
+    return TARA_OK;
+}
+
+// out[ch] += sum_{b,p} g[b][ch][p] w[b][p] / sig[b]   (w, sig optional), for ch < c.
+int chan_reduce(tara_fno* f, const float* g, int c, int nb, const float* w, const float* sig, float* out)
+{
+    hipLaunchKernelGGL(k_chan_dot, dim3(c, RED_SPLIT), dim3(256), 0, f->st, g, w, c, f->np, nb, sig,
+                       P<float>(f->part));
+    hipLaunchKernelGGL(k_sum_parts, dim3(fno_grid((size_t)c)), dim3(256), 0, f->st, P<float>(f->part), RED_SPLIT,
+                       (size_t)c, out);
+    TARA_HIP(hipGetLastError());
+    return TARA_OK;
+}
+
 // ------------------------------------------------------------------ forward
 // out: prediction (device, [nb][n][n]).  train: keep what backward() needs (FP32
 // activations, pre-activations, spectra of every layer); without it only the
 // operands of the next product are written.
+//
+// Per layer, with F = nb C and bc = b C + c:
+//   T1[y][q][bc]     q = p MX + kx (p: 0 real, 1 imaginary part)   x transform
+//   X[p][mode][bc]   mode = ky_r MX + kx                            y transform
+//   Y[p][mode][bc]                                                  per-mode contraction
+//   Z[bc][y][q]                                                     inverse y transform
+//   Kv[bc][y][x]                                                    inverse x transform
 int forward(tara_fno* f, int nb, const float* om, float* out, bool train)
 {
-    const int n = f->n, c = f->c, c2 = 2 * c, mx = f->mx, my = f->my, M = (int)f->modes;
-    const size_t np = f->np, field = (size_t)nb * c * np, xs = (size_t)M * nb * c2;
+    const int n = f->n, c = f->c, mx = f->mx, my = f->my, M = (int)f->modes, q2 = 2 * mx;
+    const long F = (long)nb * c, c4 = 4L * c * c;
+    const size_t np = f->np, field = (size_t)F * np, sp = 2 * (size_t)M * F;
     const bool bf = f->prec == 0;
     const int dt = bf ? GEMM_BF16 : GEMM_F32;
     float* par = f->par.as<float>();
     int rc;
-    hipLaunchKernelGGL(k_rms, dim3(nb), dim3(256), 0, f->st, om, P<float>(f->sig), np);
+    if ((rc = rms(f, nb, om))) return rc;
     const bool keep32 = train || !bf;  // FP32 activations needed
     hipLaunchKernelGGL(k_lift, dim3(fno_grid(field)), dim3(256), 0, f->st, om, P<float>(f->sig), par + f->o_wl,
                        par + f->o_bl, keep32 ? P<float>(f->v) : nullptr, bf ? P<uint16_t>(f->vb) : nullptr, nb, c,
                        np);
+    float* Y = P<float>(f->y);
+    float* Z = P<float>(f->zf);
+    float* kv = P<float>(f->kv);
     for (int l = 0; l < f->layers; ++l) {
         // without training, layer activations alternate between two slots
-        const size_t vo = (size_t)(train ? l : l % 2) * f->field, vn = (size_t)(train ? l + 1 : (l + 1) % 2) * f->field;
+        const size_t vo = (size_t)(train ? l : l % 2) * f->field;
+        const size_t vn = (size_t)(train ? l + 1 : (l + 1) % 2) * f->field;
         const void* vin = bf ? (const void*)P<uint16_t>(f->vb, vo) : (const void*)P<float>(f->v, vo);
-        // 1. along x: T1[b][c][part][y][kx] = sum_x v[b][c][y][x] Fx_part[x][kx]
-        for (int part = 0; part < 2; ++part) {
+        {  // 1. along x, per row y: T1[y][q][bc] = sum_x FxT[q][x] v[bc][y][x]
             GemmArgs g;
-            g.m = n, g.n = mx, g.k = n;
-            g.a = vin, g.sam = n, g.sak = 1;
-            g.b = bf ? (part ? f->fxsb.p : f->fxcb.p) : (part ? f->fxs.p : f->fxc.p), g.sbk = mx, g.sbn = 1;
-            g.c = bf ? (void*)(P<uint16_t>(f->t1) + (size_t)part * n * mx)
-                     : (void*)(P<float>(f->t1) + (size_t)part * n * mx);
-            g.scm = mx, g.scn = 1;
-            g.nb1 = c, g.sa1 = (long)np, g.sc1 = 2L * n * mx;
-            g.nb2 = nb, g.sa2 = (long)c * np, g.sc2 = (long)c * 2 * n * mx;
+            g.m = q2, g.n = F, g.k = n;
+            g.a = bf ? f->fxtb.p : f->fxt.p, g.sam = n, g.sak = 1;
+            g.b = vin, g.sbk = 1, g.sbn = (long)np;
+            g.c = f->t1.p, g.scm = F, g.scn = 1;
+            g.nb1 = n, g.sb1 = n, g.sc1 = (long)q2 * F;
             if ((rc = gemm(g, dt, bf ? GEMM_STORE_BF16 : GEMM_STORE_F32, f->st))) return rc;
         }
-        // 2. along y: X[(ky_r, kx)][b][part C + c] = sum_y' FyP[ky_r][y'] T1[b][c][y'][kx], y' over (re, im) rows
-        float* X = P<float>(f->x, (size_t)(train ? l : 0) * M * f->maxb * c2);
-        for (int part = 0; part < 2; ++part) {
+        float* X = P<float>(f->x, (size_t)(train ? l : 0) * 2 * M * f->maxb * c);
+        {  // 2. along y, per kx: X[p'][ky_r][kx][bc] = sum_{y,p} Fy2[p' MY + ky_r][2y + p] T1[y][p MX + kx][bc]
             GemmArgs g;
-            g.m = my, g.n = mx, g.k = 2L * n;
-            g.a = bf ? (part ? f->fyimb.p : f->fyreb.p) : (part ? f->fyim.p : f->fyre.p), g.sam = 2L * n, g.sak = 1;
-            g.b = f->t1.p, g.sbk = mx, g.sbn = 1;
-            g.c = X + (size_t)part * c, g.scm = (long)mx * nb * c2, g.scn = (long)nb * c2;
-            g.nb1 = c, g.sb1 = 2L * n * mx, g.sc1 = 1;
-            g.nb2 = nb, g.sb2 = (long)c * 2 * n * mx, g.sc2 = c2;
+            g.m = 2 * my, g.n = F, g.k = 2L * n;
+            g.a = bf ? f->fy2b.p : f->fy2.p, g.sam = 2L * n, g.sak = 1;
+            g.b = f->t1.p, g.sbk = (long)mx * F, g.sbn = 1;
+            g.c = X, g.scm = (long)mx * F, g.scn = 1;
+            g.nb1 = mx, g.sb1 = F, g.sc1 = F;
             if ((rc = gemm(g, dt, GEMM_STORE_F32, f->st))) return rc;
         }
-        if (bf) hipLaunchKernelGGL(k_to_bf16, dim3(fno_grid(xs)), dim3(256), 0, f->st, X, P<uint16_t>(f->xb), xs);
-        // 3. per-mode contraction: Y[m] = X[m] Rs[m]
-        {
+        if (bf) hipLaunchKernelGGL(k_to_bf16, dim3(fno_grid(sp)), dim3(256), 0, f->st, X, P<uint16_t>(f->xb), sp);
+        // 3. per mode: Y[p'][mode][b][o] = sum_p sum_c X[p][mode][b][c] Rs[mode][p C + c][p' C + o]
+        for (int p = 0; p < 2; ++p) {
             GemmArgs g;
-            g.m = nb, g.n = c2, g.k = c2;
-            g.a = bf ? (const void*)P<uint16_t>(f->xb) : (const void*)X, g.sam = c2, g.sak = 1;
-            g.b = bf ? (const void*)P<uint16_t>(f->rsb, (size_t)l * M * c2 * c2)
-                     : (const void*)P<float>(f->rs, (size_t)l * M * c2 * c2);
-            g.sbk = c2, g.sbn = 1;
-            g.c = f->y.p, g.scm = c2, g.scn = 1;
-            g.nb1 = M, g.sa1 = (long)nb * c2, g.sb1 = (long)c2 * c2, g.sc1 = (long)nb * c2;
-            if ((rc = gemm(g, dt, bf ? GEMM_STORE_BF16 : GEMM_STORE_F32, f->st))) return rc;
+            g.m = nb, g.n = c, g.k = c;
+            g.a = bf ? (const void*)(P<uint16_t>(f->xb) + (size_t)p * M * F) : (const void*)(X + (size_t)p * M * F);
+            g.sam = c, g.sak = 1;
+            g.b = bf ? (const void*)P<uint16_t>(f->rsb, (size_t)l * M * c4 + (size_t)p * c * 2 * c)
+                     : (const void*)P<float>(f->rs, (size_t)l * M * c4 + (size_t)p * c * 2 * c);
+            g.sbk = 2L * c, g.sbn = 1;
+            g.c = Y, g.scm = c, g.scn = 1;
+            g.nb1 = M, g.sa1 = F, g.sb1 = c4, g.sc1 = F;
+            g.nb2 = 2, g.sb2 = c, g.sc2 = (long)M * F;
+            if ((rc = gemm(g, dt, p == 0 ? GEMM_STORE_F32 : GEMM_ACCUM_F32, f->st))) return rc;
         }
-        // 4. inverse along y: Z[b][c][y][q MX + kx]; Zre = Gc Yre - Gs Yim, Zim = Gs Yre + Gc Yim
-        float* Z = P<float>(f->zf);
-        for (int q = 0; q < 2; ++q)
-            for (int part = 0; part < 2; ++part) {
-                const bool use_c = (q == part);
-                GemmArgs g;
-                g.m = n, g.n = mx, g.k = my;
-                g.a = bf ? (use_c ? f->gcb.p : f->gsb.p) : (use_c ? f->gc.p : f->gs.p), g.sam = my, g.sak = 1;
-                g.b = bf ? (const void*)(P<uint16_t>(f->y) + (size_t)part * c)
-                         : (const void*)(P<float>(f->y) + (size_t)part * c);
-                g.sbk = (long)mx * nb * c2, g.sbn = (long)nb * c2;
-                g.c = Z + (size_t)q * mx, g.scm = 2L * mx, g.scn = 1;
-                g.nb1 = c, g.sb1 = 1, g.sc1 = (long)n * 2 * mx;
-                g.nb2 = nb, g.sb2 = c2, g.sc2 = (long)c * n * 2 * mx;
-                g.alpha = (q == 0 && part == 1) ? -1.f : 1.f;
-                if ((rc = gemm(g, dt, part == 0 ? GEMM_STORE_F32 : GEMM_ACCUM_F32, f->st))) return rc;
-            }
-        // 5. inverse along x: Kv[b][c][y][x] = sum_j Z[b][c][y][j] Hx[j][x]  (FP32)
-        float* kv = P<float>(f->kv);
-        {
+        if (bf) hipLaunchKernelGGL(k_to_bf16, dim3(fno_grid(sp)), dim3(256), 0, f->st, Y, P<uint16_t>(f->yb), sp);
+        {  // 4. inverse along y, per kx: Z[bc][y][p' MX + kx] = sum_{p,ky_r} G2[2y + p'][p MY + ky_r] Y[p][ky_r][kx][bc]
             GemmArgs g;
-            g.m = (long)nb * c * n, g.n = n, g.k = 2L * mx;
-            g.a = Z, g.sam = 2L * mx, g.sak = 1;
+            g.m = 2L * n, g.n = F, g.k = 2 * my;
+            g.a = bf ? f->g2b.p : f->g2.p, g.sam = 2 * my, g.sak = 1;
+            g.b = bf ? (const void*)P<uint16_t>(f->yb) : (const void*)Y, g.sbk = (long)mx * F, g.sbn = 1;
+            g.c = Z, g.scm = mx, g.scn = (long)n * q2;
+            g.nb1 = mx, g.sb1 = F, g.sc1 = 1;
+            if ((rc = gemm(g, dt, GEMM_STORE_F32, f->st))) return rc;
+        }
+        {  // 5. inverse along x (FP32): Kv[bc][y][x] = sum_j Z[bc][y][j] Hx[j][x]
+            GemmArgs g;
+            g.m = F * n, g.n = n, g.k = q2;
+            g.a = Z, g.sam = q2, g.sak = 1;
             g.b = f->hx.p, g.sbk = n, g.sbn = 1;
             g.c = kv, g.scm = n, g.scn = 1;
             if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
         }
-        // 6. channel mixing: Kv[b][o][p] += sum_i W[o][i] v[b][i][p]
-        {
+        {  // 6. channel mixing: Kv[b][o][p] += sum_i W[o][i] v[b][i][p]
             GemmArgs g;
             g.m = c, g.n = (long)np, g.k = c;
             g.a = bf ? (const void*)(P<uint16_t>(f->wb) + f->o_w + (size_t)l * c * c)
@@ -527,11 +533,12 @@ int forward(tara_fno* f, int nb, const float* om, float* out, bool train)
 }
 
 // ------------------------------------------------------------------ backward
-// Gradients of the mean relative L2 loss into f->grad, after forward() on the same input.
+// Gradients of the mean relative L2 loss into f->grad, after forward(train) on the same input.
 int backward(tara_fno* f, int nb, const float* om, const float* tgt)
 {
-    const int n = f->n, c = f->c, c2 = 2 * c, mx = f->mx, my = f->my, M = (int)f->modes, H = f->hdim;
-    const size_t np = f->np, field = (size_t)nb * c * np;
+    const int n = f->n, c = f->c, mx = f->mx, my = f->my, M = (int)f->modes, H = f->hdim, q2 = 2 * mx;
+    const long F = (long)nb * c, c4 = 4L * c * c;
+    const size_t np = f->np, field = (size_t)F * np;
     float* par = f->par.as<float>();
     float* gr = f->grad.as<float>();
     const float* sig = P<float>(f->sig);
@@ -539,11 +546,14 @@ int backward(tara_fno* f, int nb, const float* om, const float* tgt)
     TARA_HIP(hipMemsetAsync(f->grad.p, 0, f->grad.bytes, f->st));
     hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, f->st, P<float>(f->out), tgt, sig, P<float>(f->dproj),
                        P<float>(f->loss), nb, np);
-    hipLaunchKernelGGL(k_proj2_wgrad, dim3(H + 1), dim3(256), 0, f->st, P<float>(f->hb), P<float>(f->dproj),
-                       gr + f->o_w2, gr + f->o_b2, nb, H, np);
+    TARA_HIP(hipGetLastError());
+    // projection head: dw2[j] = sum h[b][j] dproj[b], db2 = sum dproj
+    if ((rc = chan_reduce(f, P<float>(f->hb), H, nb, P<float>(f->dproj), nullptr, gr + f->o_w2)) ||
+        (rc = chan_reduce(f, P<float>(f->dproj), 1, nb, nullptr, nullptr, gr + f->o_b2)))
+        return rc;
     hipLaunchKernelGGL(k_proj2_back, dim3(fno_grid((size_t)nb * H * np)), dim3(256), 0, f->st, P<float>(f->dproj),
                        par + f->o_w2, P<float>(f->hpre), P<float>(f->dhpre), nb, H, np);
-    hipLaunchKernelGGL(k_bias_grad, dim3(H), dim3(256), 0, f->st, P<float>(f->dhpre), gr + f->o_b1, nb, H, np);
+    if ((rc = chan_reduce(f, P<float>(f->dhpre), H, nb, nullptr, nullptr, gr + f->o_b1))) return rc;
     const size_t vl = (size_t)f->layers * f->field;
     // dW1[j][c] += sum_{b,p} dhpre[b][j][p] v[b][c][p]
     if ((rc = wgrad(f, H, c, (long)np, P<float>(f->dhpre), (long)np, 1, (long)H * np, P<float>(f->v, vl), 1,
@@ -560,13 +570,17 @@ int backward(tara_fno* f, int nb, const float* om, const float* tgt)
         g.nb1 = nb, g.sb1 = (long)H * np, g.sc1 = (long)c * np;
         if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
     }
+    float* dz = P<float>(f->dz);
+    float* dy = P<float>(f->dy);
+    float* dx = P<float>(f->dx);
+    float* dt1 = P<float>(f->dt1);
     for (int l = f->layers - 1; l >= 0; --l) {
         const size_t vo = (size_t)l * f->field;
         if (l < f->layers - 1)
             hipLaunchKernelGGL(k_act_back, dim3(fno_grid(field)), dim3(256), 0, f->st, dv,
                                P<float>(f->pre, (size_t)l * f->field), field);
         // dv now holds d(pre-activation of layer l)
-        hipLaunchKernelGGL(k_bias_grad, dim3(c), dim3(256), 0, f->st, dv, gr + f->o_b + (size_t)l * c, nb, c, np);
+        if ((rc = chan_reduce(f, dv, c, nb, nullptr, nullptr, gr + f->o_b + (size_t)l * c))) return rc;
         // channel mixing: dW[o][i] += sum dpre[b][o][p] v[b][i][p]; dv_in = W^T dpre
         if ((rc = wgrad(f, c, c, (long)np, dv, (long)np, 1, (long)c * np, P<float>(f->v, vo), 1, (long)np,
                         (long)c * np, nb, gr + f->o_w + (size_t)l * c * c)))
@@ -580,81 +594,72 @@ int backward(tara_fno* f, int nb, const float* om, const float* tgt)
             g.nb1 = nb, g.sb1 = (long)c * np, g.sc1 = (long)c * np;
             if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
         }
-        // spectral branch, backwards.  dZ = dpre Hx^T
-        float* dz = P<float>(f->dz);
-        {
+        {  // spectral branch, backwards.  dZ[y][j][bc] = sum_x Hx[j][x] dpre[bc][y][x]
             GemmArgs g;
-            g.m = (long)nb * c * n, g.n = 2L * mx, g.k = n;
-            g.a = dv, g.sam = n, g.sak = 1;
-            g.b = f->hx.p, g.sbk = 1, g.sbn = n;
-            g.c = dz, g.scm = 2L * mx, g.scn = 1;
+            g.m = q2, g.n = F, g.k = n;
+            g.a = f->hx.p, g.sam = n, g.sak = 1;
+            g.b = dv, g.sbk = 1, g.sbn = (long)np;
+            g.c = dz, g.scm = F, g.scn = 1;
+            g.nb1 = n, g.sb1 = n, g.sc1 = (long)q2 * F;
             if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
         }
-        // dYre = Gc^T dZre + Gs^T dZim, dYim = -Gs^T dZre + Gc^T dZim  (layout [mode][b][2C])
-        float* dy = P<float>(f->dy);
-        for (int part = 0; part < 2; ++part)
-            for (int q = 0; q < 2; ++q) {
-                const bool use_c = (q == part);
-                GemmArgs g;
-                g.m = my, g.n = mx, g.k = n;
-                g.a = use_c ? f->gc.p : f->gs.p, g.sam = 1, g.sak = my;
-                g.b = dz + (size_t)q * mx, g.sbk = 2L * mx, g.sbn = 1;
-                g.c = dy + (size_t)part * c, g.scm = (long)mx * nb * c2, g.scn = (long)nb * c2;
-                g.nb1 = c, g.sb1 = (long)n * 2 * mx, g.sc1 = 1;
-                g.nb2 = nb, g.sb2 = (long)c * n * 2 * mx, g.sc2 = c2;
-                g.alpha = (part == 1 && q == 0) ? -1.f : 1.f;
-                if ((rc = gemm(g, GEMM_F32, q == 0 ? GEMM_STORE_F32 : GEMM_ACCUM_F32, f->st))) return rc;
-            }
-        // dRs[m] = X[m]^T dY[m] ; dX[m] = dY[m] Rs[m]^T
-        const float* X = P<float>(f->x, (size_t)l * M * f->maxb * c2);
-        float* drs = P<float>(f->drs);
-        {
+        {  // dY[p][ky_r][kx][bc] = sum_{y,p'} G2[2y + p'][p MY + ky_r] dZ[y][p' MX + kx][bc]
             GemmArgs g;
-            g.m = c2, g.n = c2, g.k = nb;
-            g.a = X, g.sam = 1, g.sak = c2;
-            g.b = dy, g.sbk = c2, g.sbn = 1;
-            g.c = drs, g.scm = c2, g.scn = 1;
-            g.nb1 = M, g.sa1 = (long)nb * c2, g.sb1 = (long)nb * c2, g.sc1 = (long)c2 * c2;
+            g.m = 2 * my, g.n = F, g.k = 2L * n;
+            g.a = f->g2.p, g.sam = 1, g.sak = 2 * my;
+            g.b = dz, g.sbk = (long)mx * F, g.sbn = 1;
+            g.c = dy, g.scm = (long)mx * F, g.scn = 1;
+            g.nb1 = mx, g.sb1 = F, g.sc1 = F;
+            if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
+        }
+        const float* X = P<float>(f->x, (size_t)l * 2 * M * f->maxb * c);
+        float* drs = P<float>(f->drs);
+        for (int p = 0; p < 2; ++p) {  // dRs[mode][p C + c][p' C + o] = sum_b X[p][mode][b][c] dY[p'][mode][b][o]
+            GemmArgs g;
+            g.m = c, g.n = c, g.k = nb;
+            g.a = X + (size_t)p * M * F, g.sam = 1, g.sak = c;
+            g.b = dy, g.sbk = c, g.sbn = 1;
+            g.c = drs + (size_t)p * c * 2 * c, g.scm = 2L * c, g.scn = 1;
+            g.nb1 = M, g.sa1 = F, g.sb1 = F, g.sc1 = c4;
+            g.nb2 = 2, g.sb2 = (long)M * F, g.sc2 = c;
             if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
         }
         hipLaunchKernelGGL(k_fold_drs, dim3(fno_grid((size_t)M * c * c)), dim3(256), 0, f->st, drs,
                            gr + f->o_r + (size_t)l * f->r_sz, M, c);
-        float* dx = P<float>(f->dx);
-        {
+        for (int pp = 0; pp < 2; ++pp) {  // dX[p][mode][b][c] = sum_p' sum_o dY[p'][mode][b][o] Rs[mode][p C + c][p' C + o]
             GemmArgs g;
-            g.m = nb, g.n = c2, g.k = c2;
-            g.a = dy, g.sam = c2, g.sak = 1;
-            g.b = P<float>(f->rs, (size_t)l * M * c2 * c2), g.sbk = 1, g.sbn = c2;
-            g.c = dx, g.scm = c2, g.scn = 1;
-            g.nb1 = M, g.sa1 = (long)nb * c2, g.sb1 = (long)c2 * c2, g.sc1 = (long)nb * c2;
+            g.m = nb, g.n = c, g.k = c;
+            g.a = dy + (size_t)pp * M * F, g.sam = c, g.sak = 1;
+            g.b = P<float>(f->rs, (size_t)l * M * c4 + (size_t)pp * c), g.sbk = 1, g.sbn = 2L * c;
+            g.c = dx, g.scm = c, g.scn = 1;
+            g.nb1 = M, g.sa1 = F, g.sb1 = c4, g.sc1 = F;
+            g.nb2 = 2, g.sb2 = 2L * c * c, g.sc2 = (long)M * F;
+            if ((rc = gemm(g, GEMM_F32, pp == 0 ? GEMM_STORE_F32 : GEMM_ACCUM_F32, f->st))) return rc;
+        }
+        {  // dT1[y][p MX + kx][bc] = sum_{p',ky_r} Fy2[p' MY + ky_r][2y + p] dX[p'][ky_r][kx][bc]
+            GemmArgs g;
+            g.m = 2L * n, g.n = F, g.k = 2 * my;
+            g.a = f->fy2.p, g.sam = 1, g.sak = 2L * n;
+            g.b = dx, g.sbk = (long)mx * F, g.sbn = 1;
+            g.c = dt1, g.scm = (long)mx * F, g.scn = 1;
+            g.nb1 = mx, g.sb1 = F, g.sc1 = F;
             if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
         }
-        // dT1[b][c][y'][kx] = sum_part sum_ky_r FyP[ky_r][y'] dX_part[(ky_r, kx)][b][c]
-        float* dt1 = P<float>(f->dt1);
-        for (int part = 0; part < 2; ++part) {
+        {  // dv_in[bc][y][x] += sum_q dT1[y][q][bc] FxT[q][x], per row y
             GemmArgs g;
-            g.m = 2L * n, g.n = mx, g.k = my;
-            g.a = part ? f->fyim.p : f->fyre.p, g.sam = 1, g.sak = 2L * n;
-            g.b = dx + (size_t)part * c, g.sbk = (long)mx * nb * c2, g.sbn = (long)nb * c2;
-            g.c = dt1, g.scm = mx, g.scn = 1;
-            g.nb1 = c, g.sb1 = 1, g.sc1 = 2L * n * mx;
-            g.nb2 = nb, g.sb2 = c2, g.sc2 = (long)c * 2 * n * mx;
-            if ((rc = gemm(g, GEMM_F32, part == 0 ? GEMM_STORE_F32 : GEMM_ACCUM_F32, f->st))) return rc;
-        }
-        // dv_in += dT1_part Fx_part^T
-        for (int part = 0; part < 2; ++part) {
-            GemmArgs g;
-            g.m = n, g.n = n, g.k = mx;
-            g.a = dt1 + (size_t)part * n * mx, g.sam = mx, g.sak = 1;
-            g.b = part ? f->fxs.p : f->fxc.p, g.sbk = 1, g.sbn = mx;
-            g.c = dv2, g.scm = n, g.scn = 1;
-            g.nb1 = c, g.sa1 = 2L * n * mx, g.sc1 = (long)np;
-            g.nb2 = nb, g.sa2 = (long)c * 2 * n * mx, g.sc2 = (long)c * np;
+            g.m = F, g.n = n, g.k = q2;
+            g.a = dt1, g.sam = 1, g.sak = F;
+            g.b = f->fxt.p, g.sbk = n, g.sbn = 1;
+            g.c = dv2, g.scm = (long)np, g.scn = 1;
+            g.nb1 = n, g.sa1 = (long)q2 * F, g.sc1 = n;
             if ((rc = gemm(g, GEMM_F32, GEMM_ACCUM_F32, f->st))) return rc;
         }
         std::swap(dv, dv2);
     }
-    hipLaunchKernelGGL(k_lift_grad, dim3(c), dim3(256), 0, f->st, dv, om, sig, gr + f->o_wl, gr + f->o_bl, nb, c, np);
+    // lift: dWl[c] = sum dv0 omega / sig, dbl[c] = sum dv0
+    if ((rc = chan_reduce(f, dv, c, nb, om, sig, gr + f->o_wl)) ||
+        (rc = chan_reduce(f, dv, c, nb, nullptr, nullptr, gr + f->o_bl)))
+        return rc;
     TARA_HIP(hipGetLastError());
     return TARA_OK;
 }
@@ -728,16 +733,19 @@ extern "C" int tara_fno_create(int n, int width, int mx, int my, int layers, int
     TARA_HIP(hipMemset(f->m1.p, 0, o * 4));
     TARA_HIP(hipMemset(f->m2.p, 0, o * 4));
     if ((rc = f->rs.alloc(L * M * c2 * c2 * 4)) || (rc = f->rsb.alloc(L * M * c2 * c2 * 2))) return rc;
-    // DFT matrices (angles reduced mod n in integers before scaling)
+    // Truncated DFT matrices (angles reduced mod n in integers before scaling):
+    //   FxT[q][x]            q = p MX + kx: cos(a) (p = 0), -sin(a) (p = 1), a = 2 pi kx x / n
+    //   Fy2[p' MY + r][2y+p] e^{-i b} applied to (re, im) = (p = 0, 1), output part p', b = 2 pi ky_r y / n
+    //   G2[2y + p'][p MY + r] e^{+i b} applied to (Yre, Yim), output part p'
+    //   Hx[q][x]             inverse along x with the half-spectrum weights 1 (kx = 0), 2 and 1/n^2
     const double tp = 2.0 * 3.14159265358979323846;
     auto kyv = [&](int r) { return r < my / 2 ? r : r - my; };
-    std::vector<float> fxc(n * mx), fxs(n * mx), fyre(my * 2 * n), fyim(my * 2 * n), gc(n * my), gs(n * my),
-        hx(2 * mx * n);
+    std::vector<float> fxt(2 * mx * n), fy2(2 * my * 2 * n), g2(2 * n * 2 * my), hx(2 * mx * n);
     for (int x = 0; x < n; ++x)
         for (int k = 0; k < mx; ++k) {
             const double a = tp * (double)(((long)k * x) % n) / n;
-            fxc[x * mx + k] = (float)std::cos(a);   // T1re = sum_x v cos
-            fxs[x * mx + k] = (float)-std::sin(a);  // T1im = -sum_x v sin
+            fxt[k * n + x] = (float)std::cos(a);
+            fxt[(mx + k) * n + x] = (float)-std::sin(a);
             const double w = (k == 0 ? 1.0 : 2.0) / ((double)n * n);
             hx[k * n + x] = (float)(w * std::cos(a));
             hx[(mx + k) * n + x] = (float)(-w * std::sin(a));
@@ -745,31 +753,34 @@ extern "C" int tara_fno_create(int n, int width, int mx, int my, int layers, int
     for (int r = 0; r < my; ++r)
         for (int y = 0; y < n; ++y) {
             const long ph = (((long)kyv(r) * y) % n + n) % n;
-            const double a = tp * (double)ph / n, cs = std::cos(a), sn = std::sin(a);
-            // e^{-i a} (T1re + i T1im): re = cos T1re + sin T1im, im = -sin T1re + cos T1im
-            fyre[r * 2 * n + y] = (float)cs;
-            fyre[r * 2 * n + n + y] = (float)sn;
-            fyim[r * 2 * n + y] = (float)-sn;
-            fyim[r * 2 * n + n + y] = (float)cs;
-            gc[y * my + r] = (float)cs;  // e^{+i a}
-            gs[y * my + r] = (float)sn;
+            const double a = tp * (double)ph / n;
+            const float cs = (float)std::cos(a), sn = (float)std::sin(a);
+            // forward: re' = cos re + sin im, im' = -sin re + cos im
+            fy2[(size_t)r * 2 * n + 2 * y] = cs;
+            fy2[(size_t)r * 2 * n + 2 * y + 1] = sn;
+            fy2[(size_t)(my + r) * 2 * n + 2 * y] = -sn;
+            fy2[(size_t)(my + r) * 2 * n + 2 * y + 1] = cs;
+            // inverse: re' = cos re - sin im, im' = sin re + cos im
+            g2[(size_t)(2 * y) * 2 * my + r] = cs;
+            g2[(size_t)(2 * y) * 2 * my + my + r] = -sn;
+            g2[(size_t)(2 * y + 1) * 2 * my + r] = sn;
+            g2[(size_t)(2 * y + 1) * 2 * my + my + r] = cs;
         }
-    if ((rc = upload(f->fxc, fxc)) || (rc = upload(f->fxs, fxs)) || (rc = upload(f->fyre, fyre)) ||
-        (rc = upload(f->fyim, fyim)) || (rc = upload(f->gc, gc)) || (rc = upload(f->gs, gs)) ||
-        (rc = upload(f->hx, hx)) || (rc = upload_bf(f->fxcb, fxc)) || (rc = upload_bf(f->fxsb, fxs)) ||
-        (rc = upload_bf(f->fyreb, fyre)) || (rc = upload_bf(f->fyimb, fyim)) || (rc = upload_bf(f->gcb, gc)) ||
-        (rc = upload_bf(f->gsb, gs)))
+    if ((rc = upload(f->fxt, fxt)) || (rc = upload(f->fy2, fy2)) || (rc = upload(f->g2, g2)) ||
+        (rc = upload(f->hx, hx)) || (rc = upload_bf(f->fxtb, fxt)) || (rc = upload_bf(f->fy2b, fy2)) ||
+        (rc = upload_bf(f->g2b, g2)))
         return rc;
     // activations
     const size_t B = max_batch, np = f->np;
     f->field = B * c * np;
     const size_t field = f->field, hfield = B * H * np, spec = M * B * c2, t1 = B * c * 2 * n * mx;
-    // weight-gradient partial products: up to max(256 chunks, one per sample) x (H x C)
-    const size_t parts = std::max<size_t>(512, 2 * B) * H * c;
+    // partial sums: weight gradients, up to max(512 chunks, two per sample) x (H x C);
+    // reductions, RED_SPLIT x max(H, B)
+    const size_t parts = std::max(std::max<size_t>(512, 2 * B) * H * c, (size_t)RED_SPLIT * std::max<size_t>(H, B));
     if ((rc = f->sig.alloc(B * 4)) || (rc = f->v.alloc((L + 1) * field * 4)) ||
         (rc = f->vb.alloc((L + 1) * field * 2)) || (rc = f->pre.alloc(L * field * 4)) ||
         (rc = f->x.alloc(L * spec * 4)) || (rc = f->xb.alloc(spec * 2)) || (rc = f->t1.alloc(t1 * 4)) ||
-        (rc = f->y.alloc(spec * 4)) || (rc = f->zf.alloc(t1 * 4)) ||
+        (rc = f->y.alloc(spec * 4)) || (rc = f->yb.alloc(spec * 2)) || (rc = f->zf.alloc(t1 * 4)) ||
         (rc = f->kv.alloc(std::max(field, hfield) * 4)) || (rc = f->hpre.alloc(hfield * 4)) ||
         (rc = f->hb.alloc(hfield * 4)) || (rc = f->out.alloc(B * np * 4)) || (rc = f->dproj.alloc(B * np * 4)) ||
         (rc = f->loss.alloc(B * 4)) || (rc = f->dva.alloc(field * 4)) || (rc = f->dvb.alloc(field * 4)) ||

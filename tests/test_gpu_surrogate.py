@@ -1,0 +1,102 @@
+"""The surrogate workflow on a small grid: FP64 ensemble data, training on the
+device, and rollouts re-checked by the FP64 solver.
+
+* The trained network's one-step error on held-out seeds is below that of the
+  method without it (persistence, omega(t + T) = omega(t)) and of the untrained
+  network, by at least a fifth.
+* The checker's error for a step equals the relative L2 distance between the
+  surrogate's step and an independent FP64 solve from the same state.
+* With a tolerance no step meets, every check hands its step to the solver:
+  the checked states are the solver's results from the preceding surrogate
+  states."""
+import numpy as np
+import pytest
+
+import tara_amd as T
+from tara_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+N, NU, DT, STEPS = 64, 1e-3, 5e-4, 100
+
+
+@pytest.fixture(scope="module")
+def data():
+    print("surrogate: generating 64x64 trajectories", flush=True)
+    train, _ = S.generate(N, NU, DT, STEPS, range(1, 65), 20, k0_range=(3.0, 5.0))
+    held, _ = S.generate(N, NU, DT, STEPS, range(9001, 9005), 20, k0_range=(3.0, 5.0), k0_seed=7)
+    return train, held
+
+
+@pytest.fixture(scope="module")
+def trained(data):
+    train, held = data
+    net = T.FNO(N, 32, 12, 24, 4, max_batch=16, seed=1)
+    e_untrained = S.one_step_error(net, held)[0]
+    hist = S.train(net, train, 20, 16, lr=1e-3, lr_min=1e-4)
+    print(f"  losses {np.round(hist, 4)}", flush=True)
+    return net, e_untrained
+
+
+def test_training_beats_persistence(data, trained):
+    _, held = data
+    net, e_untrained = trained
+    e_net, e_id = S.one_step_error(net, held)
+    print(f"  one-step error: trained {e_net:.4f}, untrained {e_untrained:.4f}, persistence {e_id:.4f}", flush=True)
+    # measured: trained 0.176, untrained and persistence 0.271
+    assert e_net < 0.8 * e_id
+    assert e_net < 0.8 * e_untrained
+
+
+def _solve(state, nsteps):
+    """FP64 solver advance of one real-space float32 field (independent of the rollout's checkers)."""
+    e = T.Ensemble([N], [NU], DT, T.SCHEME_RK4)
+    out = torch.empty_like(state)
+    e.set_omega(0, state)
+    e.advance(nsteps)
+    e.get_omega(0, out)
+    e.sync()
+    e.close()
+    return out
+
+
+def test_checker_errors_match_independent_solves(data, trained):
+    _, held = data
+    net, _ = trained
+    nsteps, every = 20, 5
+    checkers = [T.Ensemble([N], [NU], DT, T.SCHEME_RK4) for _ in range(nsteps // every)]
+    traj = torch.empty((nsteps + 1, 1, N, N), device="cuda")
+    traj[0, 0].copy_(held[0, 0])
+    err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=1e9)
+    assert nf == 0
+    # without take-overs the rollout is the plain network iteration
+    plain = torch.empty_like(traj)
+    plain[0].copy_(traj[0])
+    net.rollout(1, plain, nsteps, nsteps + 1, (), STEPS, 1e9)
+    assert torch.equal(plain, traj)
+    for j in range(nsteps // every):
+        t = (j + 1) * every
+        ref = _solve(traj[t - 1, 0], STEPS)
+        want = float(torch.linalg.norm(traj[t, 0] - ref) / torch.linalg.norm(ref))
+        assert abs(err[j, 0] - want) < 1e-4 * max(want, 1e-3), (j, err[j, 0], want)
+    for c in checkers:
+        c.close()
+
+
+def test_failed_checks_hand_steps_to_the_solver(data, trained):
+    _, held = data
+    net, _ = trained
+    nsteps, every = 20, 5
+    checkers = [T.Ensemble([N], [NU], DT, T.SCHEME_RK4) for _ in range(nsteps // every)]
+    traj = torch.empty((nsteps + 1, 1, N, N), device="cuda")
+    traj[0, 0].copy_(held[0, 0])
+    err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=0.0)
+    assert nf == nsteps // every
+    assert np.all(err > 0)
+    for j in range(nsteps // every):
+        t = (j + 1) * every
+        ref = _solve(traj[t - 1, 0], STEPS)
+        assert float(torch.linalg.norm(traj[t, 0] - ref) / torch.linalg.norm(ref)) < 1e-6, j
+    for c in checkers:
+        c.close()
