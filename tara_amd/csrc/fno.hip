@@ -47,10 +47,7 @@ __device__ __forceinline__ uint16_t fno_f2bf(float f)
     if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
     return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
-__device__ __forceinline__ float gelu(float x)
-{
-    return 0.5f * x * (1.f + erff(x * 0.70710678118654752f));
-}
+// derivative of gelu(x) = x Phi(x) (erf form, as applied in the GEMM epilogues)
 __device__ __forceinline__ float gelu_grad(float x)
 {
     return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
@@ -78,44 +75,28 @@ __global__ void k_to_bf16(const float* x, uint16_t* y, size_t n)
     FNO_LOOP(n) y[i] = fno_f2bf(x[i]);
 }
 
-// v0[b][c][p] = Wl[c] om[b][p] / sig[b] + bl[c]  (null outputs are skipped)
+// v0[b][c][p] = Wl[c] om[b][p] / sig[b] + bl[c]  (null outputs are skipped);
+// grid (pixel blocks, nb * c)
 __global__ void k_lift(const float* om, const float* sig, const float* wl, const float* bl, float* v, uint16_t* vb,
-                       int nb, int c, size_t np)
+                       int c, size_t np)
 {
-    FNO_LOOP((size_t)nb * c * np)
-    {
-        const size_t p = i % np, ch = (i / np) % c, b = i / (np * c);
-        const float r = wl[ch] * (om[b * np + p] / sig[b]) + bl[ch];
-        if (v) v[i] = r;
-        if (vb) vb[i] = fno_f2bf(r);
-    }
+    const int row = blockIdx.y, b = row / c, ch = row % c;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const float r = wl[ch] * (om[(size_t)b * np + p] / sig[b]) + bl[ch];
+    const size_t i = (size_t)row * np + p;
+    if (v) v[i] = r;
+    if (vb) vb[i] = fno_f2bf(r);
 }
 
-// pre = acc + bias[c]; v = act ? gelu(pre) : pre.  Null outputs are skipped.
-__global__ void k_bias_act(const float* acc, const float* bias, float* pre, float* v, uint16_t* vb, int c, size_t np,
-                           size_t total, int act)
+// out[b][p] = om[b][p] + sig[b] (out[b][p] + b2); grid (pixel blocks, nb)
+__global__ void k_finalize(float* out, const float* om, const float* sig, const float* b2, size_t np)
 {
-    FNO_LOOP(total)
-    {
-        const float p = acc[i] + bias[(i / np) % c];
-        if (pre) pre[i] = p;
-        const float r = act ? gelu(p) : p;
-        if (v) v[i] = r;
-        if (vb) vb[i] = fno_f2bf(r);
-    }
-}
-
-// out[b][p] = om[b][p] + sig[b] (sum_j w2[j] h[b][j][p] + b2)
-__global__ void k_proj2(const float* h, const float* w2, const float* b2, const float* om, const float* sig,
-                        float* out, int nb, int hdim, size_t np)
-{
-    FNO_LOOP((size_t)nb * np)
-    {
-        const size_t b = i / np, p = i % np;
-        float s = 0.f;
-        for (int j = 0; j < hdim; ++j) s += w2[j] * h[(b * hdim + j) * np + p];
-        out[i] = om[i] + sig[b] * (s + b2[0]);
-    }
+    const int b = blockIdx.y;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const size_t i = (size_t)b * np + p;
+    out[i] = om[i] + sig[b] * (out[i] + b2[0]);
 }
 
 // Rs[m][k][n] (2C x 2C) = [[Cr, Ci], [-Ci, Cr]] from Cr = R[m][0], Ci = R[m][1] ([in][out]),
@@ -311,10 +292,10 @@ struct tara_fno {
     size_t r_sz = 0;  // spectral weights per layer
     DevBuf par, grad, m1, m2, wb;  // parameters, gradients, Adam moments, bfloat16 copy of the parameters
     DevBuf rs, rsb;                // stacked per-mode matrices of every layer (float, bfloat16)
-    DevBuf fxt, fy2, g2, hx;  // truncated DFT matrices (float)
-    DevBuf fxtb, fy2b, g2b;   // the same in bfloat16
+    DevBuf fxt, fy2, g2, hx;     // truncated DFT matrices (float)
+    DevBuf fxtb, fy2b, g2b, hxb;  // the same in bfloat16
     // activations, sized for maxb samples
-    DevBuf sig, v, vb, pre, x, xb, t1, y, yb, zf, kv, hpre, hb, out, dproj, loss;
+    DevBuf sig, v, vb, pre, x, xb, t1, y, yb, zf, zb, kv, hpre, hb, out, dproj, loss;
     DevBuf dva, dvb, dz, dy, dx, dt1, drs, dhpre, part;
     int step = 0;
     hipStream_t st = nullptr;
@@ -424,16 +405,16 @@ int forward(tara_fno* f, int nb, const float* om, float* out, bool train)
 {
     const int n = f->n, c = f->c, mx = f->mx, my = f->my, M = (int)f->modes, q2 = 2 * mx;
     const long F = (long)nb * c, c4 = 4L * c * c;
-    const size_t np = f->np, field = (size_t)F * np, sp = 2 * (size_t)M * F;
+    const size_t np = f->np;
     const bool bf = f->prec == 0;
     const int dt = bf ? GEMM_BF16 : GEMM_F32;
     float* par = f->par.as<float>();
     int rc;
     if ((rc = rms(f, nb, om))) return rc;
     const bool keep32 = train || !bf;  // FP32 activations needed
-    hipLaunchKernelGGL(k_lift, dim3(fno_grid(field)), dim3(256), 0, f->st, om, P<float>(f->sig), par + f->o_wl,
-                       par + f->o_bl, keep32 ? P<float>(f->v) : nullptr, bf ? P<uint16_t>(f->vb) : nullptr, nb, c,
-                       np);
+    hipLaunchKernelGGL(k_lift, dim3((unsigned)((np + 255) / 256), (unsigned)F), dim3(256), 0, f->st, om,
+                       P<float>(f->sig), par + f->o_wl, par + f->o_bl, keep32 ? P<float>(f->v) : nullptr,
+                       bf ? P<uint16_t>(f->vb) : nullptr, c, np);
     float* Y = P<float>(f->y);
     float* Z = P<float>(f->zf);
     float* kv = P<float>(f->kv);
@@ -457,11 +438,18 @@ int forward(tara_fno* f, int nb, const float* om, float* out, bool train)
             g.m = 2 * my, g.n = F, g.k = 2L * n;
             g.a = bf ? f->fy2b.p : f->fy2.p, g.sam = 2L * n, g.sak = 1;
             g.b = f->t1.p, g.sbk = (long)mx * F, g.sbn = 1;
-            g.c = X, g.scm = (long)mx * F, g.scn = 1;
+            g.scm = (long)mx * F, g.scn = 1;
             g.nb1 = mx, g.sb1 = F, g.sc1 = F;
-            if ((rc = gemm(g, dt, GEMM_STORE_F32, f->st))) return rc;
+            int epi = GEMM_STORE_F32;
+            g.c = X;
+            if (bf && train) {
+                g.c_bf16 = f->xb.p;  // FP32 spectrum kept for the weight gradient, bf16 copy for the product
+            } else if (bf) {
+                g.c = f->xb.p;
+                epi = GEMM_STORE_BF16;
+            }
+            if ((rc = gemm(g, dt, epi, f->st))) return rc;
         }
-        if (bf) hipLaunchKernelGGL(k_to_bf16, dim3(fno_grid(sp)), dim3(256), 0, f->st, X, P<uint16_t>(f->xb), sp);
         // 3. per mode: Y[p'][mode][b][o] = sum_p sum_c X[p][mode][b][c] Rs[mode][p C + c][p' C + o]
         for (int p = 0; p < 2; ++p) {
             GemmArgs g;
@@ -474,27 +462,28 @@ int forward(tara_fno* f, int nb, const float* om, float* out, bool train)
             g.c = Y, g.scm = c, g.scn = 1;
             g.nb1 = M, g.sa1 = F, g.sb1 = c4, g.sc1 = F;
             g.nb2 = 2, g.sb2 = c, g.sc2 = (long)M * F;
+            if (bf && p == 1) g.c_bf16 = f->yb.p;  // the finished sum, rounded for the next product
             if ((rc = gemm(g, dt, p == 0 ? GEMM_STORE_F32 : GEMM_ACCUM_F32, f->st))) return rc;
         }
-        if (bf) hipLaunchKernelGGL(k_to_bf16, dim3(fno_grid(sp)), dim3(256), 0, f->st, Y, P<uint16_t>(f->yb), sp);
         {  // 4. inverse along y, per kx: Z[bc][y][p' MX + kx] = sum_{p,ky_r} G2[2y + p'][p MY + ky_r] Y[p][ky_r][kx][bc]
             GemmArgs g;
             g.m = 2L * n, g.n = F, g.k = 2 * my;
             g.a = bf ? f->g2b.p : f->g2.p, g.sam = 2 * my, g.sak = 1;
             g.b = bf ? (const void*)P<uint16_t>(f->yb) : (const void*)Y, g.sbk = (long)mx * F, g.sbn = 1;
-            g.c = Z, g.scm = mx, g.scn = (long)n * q2;
+            g.c = bf ? f->zb.p : (void*)Z, g.scm = mx, g.scn = (long)n * q2;
             g.nb1 = mx, g.sb1 = F, g.sc1 = 1;
-            if ((rc = gemm(g, dt, GEMM_STORE_F32, f->st))) return rc;
+            if ((rc = gemm(g, dt, bf ? GEMM_STORE_BF16 : GEMM_STORE_F32, f->st))) return rc;
         }
-        {  // 5. inverse along x (FP32): Kv[bc][y][x] = sum_j Z[bc][y][j] Hx[j][x]
+        {  // 5. inverse along x: Kv[bc][y][x] = sum_j Z[bc][y][j] Hx[j][x]
             GemmArgs g;
             g.m = F * n, g.n = n, g.k = q2;
-            g.a = Z, g.sam = q2, g.sak = 1;
-            g.b = f->hx.p, g.sbk = n, g.sbn = 1;
+            g.a = bf ? f->zb.p : (const void*)Z, g.sam = q2, g.sak = 1;
+            g.b = bf ? f->hxb.p : f->hx.p, g.sbk = n, g.sbn = 1;
             g.c = kv, g.scm = n, g.scn = 1;
-            if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
+            if ((rc = gemm(g, dt, GEMM_STORE_F32, f->st))) return rc;
         }
-        {  // 6. channel mixing: Kv[b][o][p] += sum_i W[o][i] v[b][i][p]
+        {  // 6. channel mixing with bias and activation in the epilogue:
+           //    v_next[b][o][p] = act(Kv[b][o][p] + sum_i W[o][i] v[b][i][p] + b[o])
             GemmArgs g;
             g.m = c, g.n = (long)np, g.k = c;
             g.a = bf ? (const void*)(P<uint16_t>(f->wb) + f->o_w + (size_t)l * c * c)
@@ -503,31 +492,41 @@ int forward(tara_fno* f, int nb, const float* om, float* out, bool train)
             g.b = vin, g.sbk = (long)np, g.sbn = 1;
             g.c = kv, g.scm = (long)np, g.scn = 1;
             g.nb1 = nb, g.sb1 = (long)c * np, g.sc1 = (long)c * np;
-            if ((rc = gemm(g, dt, GEMM_ACCUM_F32, f->st))) return rc;
+            g.bias = par + f->o_b + (size_t)l * c;
+            g.act = l < f->layers - 1 ? 1 : 0;
+            g.pre = train ? P<float>(f->pre, (size_t)l * f->field) : nullptr;
+            g.c_f32 = keep32 ? P<float>(f->v, vn) : nullptr;
+            g.c_bf16 = bf ? (void*)P<uint16_t>(f->vb, vn) : nullptr;
+            if ((rc = gemm(g, dt, GEMM_ADD_TO, f->st))) return rc;
         }
-        hipLaunchKernelGGL(k_bias_act, dim3(fno_grid(field)), dim3(256), 0, f->st, kv, par + f->o_b + (size_t)l * c,
-                           train ? P<float>(f->pre, (size_t)l * f->field) : nullptr,
-                           keep32 ? P<float>(f->v, vn) : nullptr, bf ? P<uint16_t>(f->vb, vn) : nullptr, c, np, field,
-                           l < f->layers - 1 ? 1 : 0);
     }
     // projection: hpre[b][j][p] = W1[j][c] v[b][c][p] + b1[j]; out = omega + s (w2 . gelu(hpre) + b2)
     const size_t vl = (size_t)(train ? f->layers : f->layers % 2) * f->field;
     const int H = f->hdim;
-    {
+    {  // hidden layer, bias and GELU in the epilogue: h[b][j][p] = gelu(W1[j][c] v[b][c][p] + b1[j])
         GemmArgs g;
         g.m = H, g.n = (long)np, g.k = c;
         g.a = bf ? (const void*)(P<uint16_t>(f->wb) + f->o_w1) : (const void*)(par + f->o_w1), g.sam = c, g.sak = 1;
         g.b = bf ? (const void*)P<uint16_t>(f->vb, vl) : (const void*)P<float>(f->v, vl), g.sbk = (long)np,
         g.sbn = 1;
-        g.c = f->kv.p, g.scm = (long)np, g.scn = 1;
+        g.c = f->hb.p, g.scm = (long)np, g.scn = 1;
         g.nb1 = nb, g.sb1 = (long)c * np, g.sc1 = (long)H * np;
+        g.bias = par + f->o_b1;
+        g.act = 1;
+        g.pre = train ? P<float>(f->hpre) : nullptr;
         if ((rc = gemm(g, dt, GEMM_STORE_F32, f->st))) return rc;
     }
-    const size_t hs = (size_t)nb * H * np;
-    hipLaunchKernelGGL(k_bias_act, dim3(fno_grid(hs)), dim3(256), 0, f->st, P<float>(f->kv), par + f->o_b1,
-                       train ? P<float>(f->hpre) : nullptr, P<float>(f->hb), (uint16_t*)nullptr, H, np, hs, 1);
-    hipLaunchKernelGGL(k_proj2, dim3(fno_grid((size_t)nb * np)), dim3(256), 0, f->st, P<float>(f->hb),
-                       par + f->o_w2, par + f->o_b2, om, P<float>(f->sig), out, nb, H, np);
+    {  // out[b][p] = sum_j w2[j] h[b][j][p]  (FP32)
+        GemmArgs g;
+        g.m = 1, g.n = (long)np, g.k = H;
+        g.a = par + f->o_w2, g.sam = H, g.sak = 1;
+        g.b = f->hb.p, g.sbk = (long)np, g.sbn = 1;
+        g.c = out, g.scm = (long)np, g.scn = 1;
+        g.nb1 = nb, g.sb1 = (long)H * np, g.sc1 = (long)np;
+        if ((rc = gemm(g, GEMM_F32, GEMM_STORE_F32, f->st))) return rc;
+    }
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)((np + 255) / 256), (unsigned)nb), dim3(256), 0, f->st, out, om,
+                       P<float>(f->sig), par + f->o_b2, np);
     TARA_HIP(hipGetLastError());
     return TARA_OK;
 }
@@ -768,7 +767,7 @@ extern "C" int tara_fno_create(int n, int width, int mx, int my, int layers, int
         }
     if ((rc = upload(f->fxt, fxt)) || (rc = upload(f->fy2, fy2)) || (rc = upload(f->g2, g2)) ||
         (rc = upload(f->hx, hx)) || (rc = upload_bf(f->fxtb, fxt)) || (rc = upload_bf(f->fy2b, fy2)) ||
-        (rc = upload_bf(f->g2b, g2)))
+        (rc = upload_bf(f->g2b, g2)) || (rc = upload_bf(f->hxb, hx)))
         return rc;
     // activations
     const size_t B = max_batch, np = f->np;
@@ -781,6 +780,7 @@ extern "C" int tara_fno_create(int n, int width, int mx, int my, int layers, int
         (rc = f->vb.alloc((L + 1) * field * 2)) || (rc = f->pre.alloc(L * field * 4)) ||
         (rc = f->x.alloc(L * spec * 4)) || (rc = f->xb.alloc(spec * 2)) || (rc = f->t1.alloc(t1 * 4)) ||
         (rc = f->y.alloc(spec * 4)) || (rc = f->yb.alloc(spec * 2)) || (rc = f->zf.alloc(t1 * 4)) ||
+        (rc = f->zb.alloc(t1 * 2)) ||
         (rc = f->kv.alloc(std::max(field, hfield) * 4)) || (rc = f->hpre.alloc(hfield * 4)) ||
         (rc = f->hb.alloc(hfield * 4)) || (rc = f->out.alloc(B * np * 4)) || (rc = f->dproj.alloc(B * np * 4)) ||
         (rc = f->loss.alloc(B * 4)) || (rc = f->dva.alloc(field * 4)) || (rc = f->dvb.alloc(field * 4)) ||

@@ -176,17 +176,24 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g, int avec, int bvec)
     // result register i of lane l: row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32
     const long gn = n0 + wn + lane % 32;
     if (gn >= g.n) return;
+    const bool extra = g.bias || g.act || g.pre || g.c_f32 || g.c_bf16;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const long gm = m0 + wm + 8 * (i / 4) + 4 * (lane / 32) + (i % 4);
         if (gm >= g.m) continue;
         const long off = z1 * g.sc1 + z2 * g.sc2 + gm * g.scm + gn * g.scn;
-        const float v = acc[i] * g.alpha;
-        if constexpr (EPI == GEMM_STORE_F32) {
+        float v = acc[i] * g.alpha;
+        if constexpr (EPI == GEMM_ACCUM_F32 || EPI == GEMM_ADD_TO) v += reinterpret_cast<const float*>(g.c)[off];
+        if (extra) {
+            if (g.bias) v += g.bias[gm];
+            if (g.pre) g.pre[off] = v;
+            if (g.act) v = 0.5f * v * (1.f + erff(v * 0.70710678118654752f));
+            if (g.c_f32) g.c_f32[off] = v;
+            if (g.c_bf16) reinterpret_cast<uint16_t*>(g.c_bf16)[off] = f2bf(v);
+        }
+        if constexpr (EPI == GEMM_STORE_F32 || EPI == GEMM_ACCUM_F32) {
             reinterpret_cast<float*>(g.c)[off] = v;
-        } else if constexpr (EPI == GEMM_ACCUM_F32) {
-            reinterpret_cast<float*>(g.c)[off] += v;
-        } else {
+        } else if constexpr (EPI == GEMM_STORE_BF16) {
             reinterpret_cast<uint16_t*>(g.c)[off] = f2bf(v);
         }
     }
@@ -227,6 +234,7 @@ int launch_t(const GemmArgs& g, int epi, int am, int bm, int av, int bv, dim3 gr
         case GEMM_STORE_F32: return launch_a<T, GEMM_STORE_F32>(g, am, bm, av, bv, grid, st);
         case GEMM_ACCUM_F32: return launch_a<T, GEMM_ACCUM_F32>(g, am, bm, av, bv, grid, st);
         case GEMM_STORE_BF16: return launch_a<T, GEMM_STORE_BF16>(g, am, bm, av, bv, grid, st);
+        case GEMM_ADD_TO: return launch_a<T, GEMM_ADD_TO>(g, am, bm, av, bv, grid, st);
         default: set_error("gemm: bad epilogue %d", epi); return TARA_ERR_ARG;
     }
 }
@@ -262,6 +270,7 @@ extern "C" int tara_gemm(int dtype, int epi, long m, long n, long k, const void*
                          long sc1, int nb2, long sa2, long sb2, long sc2, float alpha)
 {
     TARA_CHECK(a && b && c, TARA_ERR_ARG, "gemm: null pointer");
+    TARA_CHECK(epi >= GEMM_STORE_F32 && epi <= GEMM_STORE_BF16, TARA_ERR_ARG, "gemm: bad epilogue %d", epi);
     GemmArgs g{};
     g.m = m;
     g.n = n;

@@ -144,3 +144,33 @@ def solver_time(n, nu, dt, total_steps, seed=1, k0=5.0):
         return (time.perf_counter() - t0) * 1e3
     finally:
         s.close()
+
+
+def _main(argv=None):
+    """Times network steps on random fields (for profiling):
+    python -m tara_amd.surrogate [--batch B] [--reps R] [--train] [--n N]"""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=1)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--train", action="store_true", help="time training steps instead of inference")
+    a = ap.parse_args(argv)
+    net = FNO(a.n, 64, 16, 32, 4, max_batch=a.batch, seed=1)
+    om = torch.randn((a.batch, a.n, a.n), device="cuda")
+    tgt = 0.9 * om
+    out = torch.empty_like(om)
+    step = (lambda: net.train_step(a.batch, om, tgt, 1e-4)) if a.train else (lambda: net.forward(a.batch, om, out))
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.reps
+    print(f"{'train' if a.train else 'forward'} batch {a.batch}: {ms:.3f} ms per step", flush=True)
+
+
+if __name__ == "__main__":
+    _main()
