@@ -278,6 +278,14 @@ int tara_mhd_download(tara_mhd* ctx, double* const state[8]);
 int tara_mhd_advance(tara_mhd* ctx, int nsteps);
 int tara_mhd_sync(tara_mhd* ctx);
 int tara_mhd_destroy(tara_mhd* ctx);
+/* The reference's initial state, src/main3com.f95:290-324 (ABC flow): uniform
+ * rho0 and pressure p0, u = u0 (a sin(kf z) + cc cos(kf y), b sin(kf x) + a cos(kf z),
+ * cc sin(kf y) + b cos(kf x)), uniform B = b0 on each axis, E = p0 + rho0 u^2/2 +
+ * B^2/2, on the grid x(i) = (i-1) Lx/Nx; transformed on the device with the
+ * library's FFTs into the context's state (history zeroed).  The main's values:
+ * rho0 = 1, u0 = 0.1, b0 = 0.1 sqrt(mu_0), p0 = 1, a = b = cc = kf = 1 (:228-268). */
+int tara_mhd_ic_abc(tara_mhd* ctx, double rho0, double u0, double b0, double p0, double a, double b, double cc,
+                    double kf);
 /* Kernel timing as for tara_csh: classes 0 = inverse z+y passes, 1 = x pass,
  * 2 = forward y pass, 3 = forward z pass + assembly + update. */
 int tara_mhd_set_timing(tara_mhd* ctx, int on);

@@ -145,6 +145,7 @@ def lib():
         for n in ("tara_mhd_sync", "tara_mhd_destroy"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.tara_mhd_set_timing.argtypes = [C.c_void_p, C.c_int]
+        L.tara_mhd_ic_abc.argtypes = [C.c_void_p] + [C.c_double] * 8
         L.tara_mhd_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
         _lib = L
     return _lib
@@ -648,6 +649,10 @@ class MhdSolver:
 
     def advance(self, nsteps):
         _check(lib().tara_mhd_advance(self._h, nsteps))
+
+    def ic_abc(self, rho0=1.0, u0=0.1, b0=0.1, p0=1.0, a=1.0, b=1.0, c=1.0, kf=1.0):
+        """The reference's ABC-flow initial state (src/main3com.f95:290-324) on the device."""
+        _check(lib().tara_mhd_ic_abc(self._h, rho0, u0, b0, p0, a, b, c, kf))
 
     def sync(self):
         _check(lib().tara_mhd_sync(self._h))

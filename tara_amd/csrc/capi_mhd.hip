@@ -1,11 +1,13 @@
 // C ABI of the 3D MHD path: drop-in derive3com / ab3com on host arrays, and
 // the device-resident stepping context.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <tuple>
 
+#include "fft_api.h"
 #include "mhd3d.h"
 
 namespace tara {
@@ -192,6 +194,63 @@ extern "C" int tara_mhd_advance(tara_mhd* c, int nsteps)
 extern "C" int tara_mhd_sync(tara_mhd* c)
 {
     TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    TARA_HIP(hipStreamSynchronize(c->s));
+    return TARA_OK;
+}
+
+namespace {
+
+// Field f (reference order rho, rho ux, rho uy, rho uz, E, Bx, By, Bz) of the
+// ABC-flow state of src/main3com.f95:290-324 on the grid x(i) = (i-1) dx:
+// uniform rho0 and P0, u = U0 (A sin kz + C cos ky, B sin kx + A cos kz,
+// C sin ky + B cos kx) with wavenumber kf, uniform B = B0 on every axis,
+// E = P + rho u^2/2 + B^2/2.
+__global__ void k_abc_field(double* out, int f, int nx, int ny, int nz, double dx, double dy, double dz,
+                            double rho0, double u0, double b0, double p0, double a, double b, double cc, double kf)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+        const double x = i * dx, y = j * dy, z = k * dz;
+        const double ux = u0 * (a * sin(kf * z) + cc * cos(kf * y));
+        const double uy = u0 * (b * sin(kf * x) + a * cos(kf * z));
+        const double uz = u0 * (cc * sin(kf * y) + b * cos(kf * x));
+        double v;
+        switch (f) {
+            case 0: v = rho0; break;
+            case 1: v = rho0 * ux; break;
+            case 2: v = rho0 * uy; break;
+            case 3: v = rho0 * uz; break;
+            case 4: v = p0 + 0.5 * rho0 * (ux * ux + uy * uy + uz * uz) + 0.5 * (3.0 * b0 * b0); break;
+            default: v = b0; break;
+        }
+        out[q] = v;
+    }
+}
+
+}  // namespace
+
+extern "C" int tara_mhd_ic_abc(tara_mhd* c, double rho0, double u0, double b0, double p0, double a, double b,
+                               double cc, double kf)
+{
+    TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    const MhdPlan& p = c->plan;
+    const size_t n = (size_t)p.nx * p.ny * p.nz;
+    DevBuf real;
+    int rc;
+    if ((rc = real.alloc(n * sizeof(double)))) return rc;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    for (int f = 0; f < 8; ++f) {
+        double2* s = c->st[f].as<double2>();
+        TARA_HIP(hipMemsetAsync(s, 0, c->st[f].bytes, c->s));
+        hipLaunchKernelGGL(k_abc_field, dim3((unsigned)blocks), dim3(256), 0, c->s, real.as<double>(), f, p.nx,
+                           p.ny, p.nz, p.lx / p.nx, p.ly / p.ny, p.lz / p.nz, rho0, u0, b0, p0, a, b, cc, kf);
+        TARA_HIP(hipGetLastError());
+        if ((rc = row_r2c(real.as<double>(), s, p.nx, p.ld, p.ny * p.nz, c->s))) return rc;
+        if ((rc = col_fft(s, p.ny, p.ld, p.ld, (long)p.ld * p.ny, p.nz, -1, c->s))) return rc;
+        if ((rc = col_fft(s, p.nz, p.ld, (long)p.ld * p.ny, p.ld, p.ny, -1, c->s))) return rc;
+    }
+    for (int f = 0; f < 7; ++f) TARA_HIP(hipMemsetAsync(c->hist[f].p, 0, c->hist[f].bytes, c->s));
     TARA_HIP(hipStreamSynchronize(c->s));
     return TARA_OK;
 }

@@ -140,6 +140,32 @@ module tara_hip_mod
     type(c_ptr) function tara_last_error() bind(C, name="tara_last_error")
       import :: c_ptr
     end function
+
+    ! checkpoints (include/tara_hip.h; host I/O)
+    integer(c_int) function tara_ckpt_write(path, nx, ny, nz, nfields, step, time, fields) &
+        bind(C, name="tara_ckpt_write")
+      import :: c_int, c_long, c_double, c_ptr, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: nx, ny, nz, nfields
+      integer(c_long), value :: step
+      real(c_double), value :: time
+      type(c_ptr), intent(in) :: fields(*)
+    end function
+
+    integer(c_int) function tara_ckpt_read_header(path, nx, ny, nz, nfields, step, time) &
+        bind(C, name="tara_ckpt_read_header")
+      import :: c_int, c_long, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: nx, ny, nz, nfields
+      integer(c_long), intent(out) :: step
+      real(c_double), intent(out) :: time
+    end function
+
+    integer(c_int) function tara_ckpt_read(path, fields) bind(C, name="tara_ckpt_read")
+      import :: c_int, c_ptr, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      type(c_ptr), intent(in) :: fields(*)
+    end function
   end interface
 
 contains
@@ -159,6 +185,47 @@ contains
     end do
     write(0, '(a,a,i0,a,a)') trim(where), ': libtara_hip status ', rc, ': ', trim(text)
     error stop 1
+  end subroutine
+
+  ! File initial condition for a kept 2D main, in place of its commented reader
+  ! (src/main2coh.f95:83, 149): reads a one-field checkpoint (written by
+  ! tara_write_state2d, tara_ckpt_write or tara_amd.snapio) into omegak(Nh, Ny).
+  subroutine tara_read_state2d(path, nh, ny, omegak, step, time)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: nh, ny
+    complex(c_double_complex), target, intent(inout) :: omegak(nh, ny)
+    integer(c_long), intent(out), optional :: step
+    real(c_double), intent(out), optional :: time
+    character(kind=c_char, len=len_trim(path) + 1) :: cpath
+    integer(c_int) :: fnx, fny, fnz, nf
+    integer(c_long) :: st
+    real(c_double) :: t
+    type(c_ptr) :: p(1)
+    cpath = trim(path) // c_null_char
+    call tara_check(tara_ckpt_read_header(cpath, fnx, fny, fnz, nf, st, t), 'tara_read_state2d')
+    if (fnx / 2 + 1 /= nh .or. fny /= ny .or. fnz /= 1 .or. nf /= 1) then
+      write(0, '(a,a)') 'tara_read_state2d: field shape differs in ', trim(path)
+      error stop 1
+    end if
+    p(1) = c_loc(omegak)
+    call tara_check(tara_ckpt_read(cpath, p), 'tara_read_state2d')
+    if (present(step)) step = st
+    if (present(time)) time = t
+  end subroutine
+
+  ! The matching writer: omegak(Nx/2+1, Ny) with its step and time.
+  subroutine tara_write_state2d(path, nx, ny, omegak, step, time)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: nx, ny
+    complex(c_double_complex), target, intent(in) :: omegak(nx / 2 + 1, ny)
+    integer(c_long), intent(in) :: step
+    real(c_double), intent(in) :: time
+    character(kind=c_char, len=len_trim(path) + 1) :: cpath
+    type(c_ptr) :: p(1)
+    cpath = trim(path) // c_null_char
+    p(1) = c_loc(omegak)
+    call tara_check(tara_ckpt_write(cpath, int(nx, c_int), int(ny, c_int), 1_c_int, 1_c_int, step, time, p), &
+                    'tara_write_state2d')
   end subroutine
 
 end module tara_hip_mod

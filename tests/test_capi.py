@@ -105,3 +105,25 @@ def test_fort_text_snapshot(tmp_path):
     # i outer, j inner (src/main2csh.f95:170-178)
     assert np.allclose(rows[1], [0.0, np.pi, om[0, 1], 2 * om[0, 1], 3 * om[0, 1]])
     assert np.allclose(rows[2, :2], [np.pi / 2, 0.0])
+
+
+def test_fortran_file_ic_hook(tmp_path):
+    """tara_read_state2d / tara_write_state2d (tara_amd/fortran/tara_hip_mod.f90),
+    the file initial condition of a kept main, round-trip a checkpoint written
+    from Python (host I/O only)."""
+    import subprocess
+
+    from tara_amd import snapio
+
+    exe = os.path.join(ROOT, "tests", "fortran", "drive_ckpt")
+    if not os.path.exists(exe):
+        pytest.skip("tests/fortran/drive_ckpt not built (amdflang absent)")
+    nx, ny = 32, 16
+    rng = np.random.default_rng(4)
+    w = np.asfortranarray(rng.standard_normal((nx // 2 + 1, ny)) + 1j * rng.standard_normal((nx // 2 + 1, ny)))
+    snapio.write_checkpoint(tmp_path / "in.ckpt", [w], nx, step=7, time=1.25)
+    subprocess.run([exe, str(tmp_path / "in.ckpt"), str(tmp_path / "out.ckpt"), str(nx), str(ny)], check=True,
+                   timeout=60)
+    fields, h = snapio.read_checkpoint(tmp_path / "out.ckpt")
+    assert h["step"] == 8 and h["time"] == 1.75 and (h["nx"], h["ny"], h["nz"]) == (nx, ny, 1)
+    assert np.array_equal(fields[0], 2.0 * w)
