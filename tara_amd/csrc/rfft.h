@@ -86,21 +86,43 @@ __device__ __forceinline__ int rf_first_idx(int t, int e)
     return rf_in_idx<N, 1, EM>(t, e);
 }
 
+// Twiddles of pass L for the E/R radix-R butterflies a thread computes (none
+// in the first pass).  Loaded one pass ahead, before the LDS exchange, so that
+// their latency overlaps the exchange (the exchange barriers wait on LDS
+// operations only).
+template <int N, int L, int EM>
+struct RfTw {
+    static constexpr int E = RF<N, EM>::E;
+    static constexpr int R = fft_radix(N / L, E);
+    static constexpr int CNT = L > 1 ? (E / R) * (R - 1) : 1;
+    double2 w[CNT];
+    __device__ __forceinline__ void load(int t, const double2* tw)
+    {
+        if constexpr (L > 1) {
+            constexpr int TWO = fft_tw_offset(N, L, E);
+#pragma unroll
+            for (int u = 0; u < E / R; ++u) {
+                const int b = t + u * RF<N, EM>::T;
+                const double2* p = tw + TWO + (b % L) * (R - 1);
+#pragma unroll
+                for (int s = 1; s < R; ++s) w[u * (R - 1) + s - 1] = p[s - 1];
+            }
+        }
+    }
+};
+
 template <int N, int L, int DIR, int EM>
-__device__ __forceinline__ void rf_compute(double2 (&v)[(RF<N, EM>::E)], int t, const double2* tw)
+__device__ __forceinline__ void rf_compute(double2 (&v)[(RF<N, EM>::E)], const RfTw<N, L, EM>& tw)
 {
     constexpr int E = RF<N, EM>::E;
     constexpr int R = fft_radix(N / L, E);
-    constexpr int TWO = fft_tw_offset(N, L, E);
     static_assert(R <= E, "radix exceeds the elements a thread holds");
 #pragma unroll
     for (int u = 0; u < E / R; ++u) {
         if (L > 1) {
-            const int b = t + u * RF<N, EM>::T;
-            const double2* w = tw + TWO + (b % L) * (R - 1);
 #pragma unroll
             for (int s = 1; s < R; ++s) {
-                double2 x = w[s - 1];
+                double2 x = tw.w[u * (R - 1) + s - 1];
                 if (DIR > 0) x.y = -x.y;
                 v[u * R + s] = cmul(v[u * R + s], x);
             }
@@ -110,12 +132,15 @@ __device__ __forceinline__ void rf_compute(double2 (&v)[(RF<N, EM>::E)], int t, 
 }
 
 template <int N, int L, int DIR, int EM, bool WAVE, bool SPLIT>
-__device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw)
+__device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw,
+                                       const RfTw<N, L, EM>& cur)
 {
     constexpr int E = RF<N, EM>::E;
     constexpr int R = fft_radix(N / L, E);
-    rf_compute<N, L, DIR, EM>(v, t, tw);
+    rf_compute<N, L, DIR, EM>(v, cur);
     if constexpr (L * R < N) {
+        RfTw<N, L * R, EM> nxt;
+        nxt.load(t, tw);
         if constexpr (SPLIT) {
             // real and imaginary parts in two rounds through a buffer of
             // doubles: half the LDS of the double2 exchange
@@ -140,7 +165,7 @@ __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, doub
             for (int e = 0; e < E; ++e) v[e] = lds[rf_pad(rf_in_idx<N, L * R, EM>(t, e))];
             rf_sync<WAVE>();
         }
-        rf_rec<N, L * R, DIR, EM, WAVE, SPLIT>(v, t, lds, tw);
+        rf_rec<N, L * R, DIR, EM, WAVE, SPLIT>(v, t, lds, tw, nxt);
     }
 }
 
@@ -154,7 +179,8 @@ __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, doub
 template <int N, int DIR, int EM = 8, bool WAVE = false, bool SPLIT = false>
 __device__ __forceinline__ void rf_fft(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw)
 {
-    rf_rec<N, 1, DIR, EM, WAVE, SPLIT>(v, t, lds, tw);
+    const RfTw<N, 1, EM> first{};
+    rf_rec<N, 1, DIR, EM, WAVE, SPLIT>(v, t, lds, tw, first);
 }
 
 // Reorders registers from final (output) positions to first-pass (input)
