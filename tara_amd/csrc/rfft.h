@@ -131,7 +131,7 @@ __device__ __forceinline__ void rf_compute(double2 (&v)[(RF<N, EM>::E)], const R
     }
 }
 
-template <int N, int L, int DIR, int EM, bool WAVE, bool SPLIT>
+template <int N, int L, int DIR, int EM, bool WAVE, bool SPLIT, bool TWPRE>
 __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw,
                                        const RfTw<N, L, EM>& cur)
 {
@@ -140,7 +140,7 @@ __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, doub
     rf_compute<N, L, DIR, EM>(v, cur);
     if constexpr (L * R < N) {
         RfTw<N, L * R, EM> nxt;
-        nxt.load(t, tw);
+        if constexpr (TWPRE) nxt.load(t, tw);
         if constexpr (SPLIT) {
             // real and imaginary parts in two rounds through a buffer of
             // doubles: half the LDS of the double2 exchange
@@ -165,7 +165,8 @@ __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, doub
             for (int e = 0; e < E; ++e) v[e] = lds[rf_pad(rf_in_idx<N, L * R, EM>(t, e))];
             rf_sync<WAVE>();
         }
-        rf_rec<N, L * R, DIR, EM, WAVE, SPLIT>(v, t, lds, tw, nxt);
+        if constexpr (!TWPRE) nxt.load(t, tw);
+        rf_rec<N, L * R, DIR, EM, WAVE, SPLIT, TWPRE>(v, t, lds, tw, nxt);
     }
 }
 
@@ -175,12 +176,13 @@ __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, doub
 // (SPLIT = true: real and imaginary parts exchanged in two rounds).  tw: the
 // twiddle table of the (N, RF<N, EM>::E) schedule.  With WAVE = false it
 // contains workgroup barriers: every thread of the workgroup must call it the
-// same number of times.
-template <int N, int DIR, int EM = 8, bool WAVE = false, bool SPLIT = false>
+// same number of times.  TWPRE: fetch each pass's twiddles before the
+// preceding exchange (more live registers; off where that makes a kernel spill).
+template <int N, int DIR, int EM = 8, bool WAVE = false, bool SPLIT = false, bool TWPRE = true>
 __device__ __forceinline__ void rf_fft(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw)
 {
     const RfTw<N, 1, EM> first{};
-    rf_rec<N, 1, DIR, EM, WAVE, SPLIT>(v, t, lds, tw, first);
+    rf_rec<N, 1, DIR, EM, WAVE, SPLIT, TWPRE>(v, t, lds, tw, first);
 }
 
 // Reorders registers from final (output) positions to first-pass (input)
