@@ -66,6 +66,36 @@ def cpu_baseline(n):
                          else ", C restatement (not FFTW)")}
 
 
+def bench_2d(n, steps, warmup, nu=1e-4, dt=2e-4):
+    """Secondary line: the same RK4 step at another grid (the configuration's
+    other size, 1024^2): value, per-kernel times and the model rate."""
+    import tara_amd as T
+    from tara_amd.ic import random_phase_vorticity
+
+    s = T.CshSolver(n, n, nu, dt, T.SCHEME_RK4)
+    s.upload(random_phase_vorticity(n, seed=1, k0=4.0))
+    xshape, _ = s.tune(reps=5)
+    s.advance(warmup)
+    s.sync()
+    reps = steps * max(1, (4096 // n) ** 2 // 4)
+    t0 = time.perf_counter()
+    s.advance(reps)
+    s.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    s.set_timing(True)
+    s.advance(steps)
+    s.sync()
+    counts, kms = s.kernel_stats(3)
+    s.close()
+    S = 16.0 * (n // 2 + 1) * n
+    names = {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd"}
+    kernels = {names[c]: {"avg_ms": round(kms[c] / counts[c], 4), "launches": counts[c],
+                          "GBps": round(5 * S / (kms[c] / counts[c] * 1e-3) / 1e9, 1)} for c in range(3) if counts[c]}
+    return {"workload": f"2D incompressible hydro, RK4, {n}^2, FP64", "value": n * n / (ms * 1e-3),
+            "unit": "grid-pt steps/s", "ms_per_step": ms, "steps": reps, "xpass_shape": xshape, "kernels": kernels,
+            "step_model_bytes": 60 * S, "step_model_GBps": round(60 * S / (ms * 1e-3) / 1e9, 1)}
+
+
 def bench_mhd(steps, warmup, n=256):
     """Secondary line: 3D compressible MHD, AB2, 256^3, nu = eta = 1e-3,
     dt = 5e-4 (derive3com + ab3com on the GPU, state resident)."""
@@ -306,6 +336,7 @@ def main():
     achieved = model[dom] / (avg[dom] * 1e-3) / 1e9
     kernels = {names[c]: {"avg_ms": round(avg[c], 4), "launches": counts[c],
                           "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}
+    sec_1024 = bench_2d(1024, a.steps, a.warmup) if n != 1024 else None
     mhd = None if a.no_3d else bench_mhd(a.steps, a.warmup)
     ens = None if a.no_ensemble else bench_ensemble(min(a.steps, 20), a.warmup)
     fno = None if a.no_fno else bench_fno(a.fno_epochs)
@@ -335,6 +366,7 @@ def main():
         "step_model_GBps": round(60 * S / (ms_per_step * 1e-3) / 1e9, 1),
         "state_finite": finite,
         "cpu_baseline": cpu,
+        "secondary_2d_1024": sec_1024,
         "secondary_3d_mhd": mhd,
         "secondary_ensemble": ens,
         "secondary_fno": fno,
