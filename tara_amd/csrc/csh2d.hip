@@ -179,7 +179,7 @@ __device__ __forceinline__ void x_c2r_pair(const double2* ga, const double2* gb,
         }
         v[e] = make_double2(a.x - b.y, a.y + b.x);
     }
-    rf_fft<NX, +1, EM, false, SPLIT, false>(v, t, xl, tw);
+    rf_fft<NX, +1, EM, false, SPLIT>(v, t, xl, tw);
 }
 
 // Shape of the x pass: EM complex elements per thread (8: radix-8 passes; 16:
@@ -242,7 +242,7 @@ __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
     __syncthreads();
     constexpr int HALF = SPLIT ? RF<M, EM>::LDS / 2 : RF<M, EM>::LDS;  // upper half, complex elements
     double2* zl = xl + (t < S::TM ? 0 : HALF);
-    rf_fft<M, -1, EM, false, SPLIT, false>(z, tm, zl, tw_m);
+    rf_fft<M, -1, EM, false, SPLIT>(z, tm, zl, tw_m);
     if (t < S::TM) {
 #pragma unroll
         for (int e = 0; e < EH; ++e) xl[rf_pad(rf_final_idx<M, EM>(tm, e))] = z[e];

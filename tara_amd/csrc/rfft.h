@@ -87,14 +87,21 @@ __device__ __forceinline__ int rf_first_idx(int t, int e)
 }
 
 // Twiddles of pass L for the E/R radix-R butterflies a thread computes (none
-// in the first pass).  Loaded one pass ahead, before the LDS exchange, so that
-// their latency overlaps the exchange (the exchange barriers wait on LDS
-// operations only).
+// in the first pass).  Of the R - 1 powers w^s of a butterfly's twiddle only
+// the power-of-two ones (w, w^2, w^4, w^8) are fetched from the table; the
+// others are formed as one product of two already available powers (w^s =
+// w^(s - low) w^low, low = lowest set bit of s), at most three products deep.
+// That is log2(R) loads instead of R - 1 (the memory-latency-bound x pass
+// issued most of its loads for twiddles) at a cost of a few ulp.  Loaded one
+// pass ahead, before the LDS exchange, so that their latency overlaps the
+// exchange (the exchange barriers wait on LDS operations only).
 template <int N, int L, int EM>
 struct RfTw {
     static constexpr int E = RF<N, EM>::E;
     static constexpr int R = fft_radix(N / L, E);
-    static constexpr int CNT = L > 1 ? (E / R) * (R - 1) : 1;
+    static constexpr int LOGR = R >= 16 ? 4 : R >= 8 ? 3 : R >= 4 ? 2 : 1;
+    static_assert((1 << LOGR) == R, "radix must be a power of two");
+    static constexpr int CNT = L > 1 ? (E / R) * LOGR : 1;
     double2 w[CNT];
     __device__ __forceinline__ void load(int t, const double2* tw)
     {
@@ -105,7 +112,7 @@ struct RfTw {
                 const int b = t + u * RF<N, EM>::T;
                 const double2* p = tw + TWO + (b % L) * (R - 1);
 #pragma unroll
-                for (int s = 1; s < R; ++s) w[u * (R - 1) + s - 1] = p[s - 1];
+                for (int q = 0; q < LOGR; ++q) w[u * LOGR + q] = p[(1 << q) - 1];
             }
         }
     }
@@ -116,15 +123,28 @@ __device__ __forceinline__ void rf_compute(double2 (&v)[(RF<N, EM>::E)], const R
 {
     constexpr int E = RF<N, EM>::E;
     constexpr int R = fft_radix(N / L, E);
+    constexpr int LOGR = RfTw<N, L, EM>::LOGR;
     static_assert(R <= E, "radix exceeds the elements a thread holds");
 #pragma unroll
     for (int u = 0; u < E / R; ++u) {
-        if (L > 1) {
+        if constexpr (L > 1) {
+            double2 x[R];
 #pragma unroll
             for (int s = 1; s < R; ++s) {
-                double2 x = tw.w[u * (R - 1) + s - 1];
-                if (DIR > 0) x.y = -x.y;
-                v[u * R + s] = cmul(v[u * R + s], x);
+                const int low = s & -s;
+                if (low == s) {
+                    int q = 0;
+                    while ((1 << q) < s) ++q;
+                    x[s] = tw.w[u * LOGR + q];
+                } else {
+                    x[s] = cmul(x[s - low], x[low]);
+                }
+            }
+#pragma unroll
+            for (int s = 1; s < R; ++s) {
+                double2 y = x[s];
+                if (DIR > 0) y.y = -y.y;
+                v[u * R + s] = cmul(v[u * R + s], y);
             }
         }
         Dft<R, DIR>::run(&v[u * R]);
@@ -177,7 +197,7 @@ __device__ __forceinline__ void rf_rec(double2 (&v)[(RF<N, EM>::E)], int t, doub
 // twiddle table of the (N, RF<N, EM>::E) schedule.  With WAVE = false it
 // contains workgroup barriers: every thread of the workgroup must call it the
 // same number of times.  TWPRE: fetch each pass's twiddles before the
-// preceding exchange (more live registers; off where that makes a kernel spill).
+// preceding exchange (more live registers across the exchange).
 template <int N, int DIR, int EM = 8, bool WAVE = false, bool SPLIT = false, bool TWPRE = true>
 __device__ __forceinline__ void rf_fft(double2 (&v)[(RF<N, EM>::E)], int t, double2* lds, const double2* tw)
 {
