@@ -242,6 +242,10 @@ def bench_fno(epochs, nseeds=400, nheld=8):
     fwd_ms = (time.perf_counter() - t0) * 1e3 / reps
     chk = S.rollout(net, held, nu, dt, steps, every=10, tol=0.05, reps=2)
     free = S.rollout(net, held, nu, dt, steps, every=51, tol=0.05, reps=2)
+    # speed only: every check runs, none takes over (tolerance above any error);
+    # checks batched into one ensemble, and one ensemble per check for comparison
+    cpass = S.rollout(net, held, nu, dt, steps, every=10, tol=1e9, reps=2)
+    cpass_sep = S.rollout(net, held, nu, dt, steps, every=10, tol=1e9, reps=2, batched=False)
     nsteps = held.shape[0] - 1
     sim_time = nsteps * steps * dt
     solver_ms = S.solver_time(n, nu, dt, nsteps * steps)
@@ -270,6 +274,9 @@ def bench_fno(epochs, nseeds=400, nheld=8):
         "forward_gflop_per_sample": flops / 1e9,
         "solver_ms_per_rollout": solver_ms,
         "checked_ms_per_rollout": med(chk["ms"]), "unchecked_ms_per_rollout": med(free["ms"]),
+        "checks_without_takeover_ms_per_rollout": med(cpass["ms"]),
+        "speedup_checks_without_takeover": solver_ms / med(cpass["ms"]),
+        "checks_without_takeover_separate_ms_per_rollout": med(cpass_sep["ms"]),
     }
 
 

@@ -326,15 +326,17 @@ int tara_fno_train_step(tara_fno* f, int batch, const float* omega, const float*
 int tara_fno_rel_err(tara_fno* f, int batch, const float* a, const float* ref, float* err_host);
 /* Surrogate rollout with the FP64 solver as checker.  traj: device float
  * [nsteps+1][batch][n][n], traj[0] set by the caller; step t+1 = network(step t).
- * Every `every`-th step t+1 = (j+1)*every is re-checked: checker ensemble j
- * (runs 0..batch-1 of grid n, the solver's viscosity, dt and scheme) advances
- * traj[t] by solver_steps steps on its own streams, concurrently with the
- * rollout; nchk = nsteps / every checkers are needed.  When a check's relative
- * L2 error exceeds tol, the solver's field replaces that sample's step (j+1)*every
- * and the rollout resumes from there.  check_err (host, [nchk][batch]) receives
- * each check's error, fallbacks the number of take-overs.  use_graph replays a
- * captured network step.  Synchronous. */
-int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps, int every, int nchk, tara_ens* const* checkers,
+ * Every `every`-th step t+1 = (j+1)*every (j < nchk = nsteps / every) is
+ * re-checked: the solver (the checker ensembles' viscosity, dt and scheme, grid
+ * n) advances traj[t] by solver_steps steps.  nens = nchk: ensemble j (batch
+ * runs) runs check j on its own streams as soon as traj[t] exists, concurrently
+ * with the rollout; nens = 1: one ensemble of nchk*batch runs (run j*batch + b)
+ * advances every check together once the rollout's steps are done.  When a
+ * check's relative L2 error exceeds tol, the solver's field replaces that
+ * sample's step (j+1)*every and the rollout resumes from there.  check_err
+ * (host, [nchk][batch]) receives each check's error, fallbacks the number of
+ * take-overs.  use_graph replays a captured network step.  Synchronous. */
+int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps, int every, int nens, tara_ens* const* checkers,
                      int solver_steps, float tol, int use_graph, float* check_err, int* fallbacks);
 int tara_fno_destroy(tara_fno* f);
 

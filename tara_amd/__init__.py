@@ -396,13 +396,16 @@ class FNO:
     def rollout(self, batch, traj, nsteps, every=10, checkers=(), solver_steps=100, tol=0.05, graph=True):
         """Surrogate rollout of traj[0] (device float32 [nsteps+1][batch][n][n])
         with FP64 checks every `every` steps by the Ensemble objects in
-        `checkers` (nsteps // every of them, each with `batch` runs of this grid).
-        Returns (check errors [nchk][batch], number of solver take-overs)."""
-        nchk = len(checkers)
-        hs = (C.c_void_p * max(nchk, 1))(*[c._h for c in checkers])
+        `checkers`: nsteps // every of them with `batch` runs each (checks start
+        as their inputs appear), or one with (nsteps // every) * batch runs (all
+        checks advanced together after the rollout).  Returns (check errors
+        [nchk][batch], number of solver take-overs)."""
+        nens = len(checkers)
+        nchk = nsteps // every
+        hs = (C.c_void_p * max(nens, 1))(*[c._h for c in checkers])
         err = np.zeros(max(nchk * batch, 1), np.float32)
         nf = C.c_int()
-        _check(lib().tara_fno_rollout(self._h, batch, _dptr(traj), nsteps, every, nchk, hs, solver_steps, tol,
+        _check(lib().tara_fno_rollout(self._h, batch, _dptr(traj), nsteps, every, nens, hs, solver_steps, tol,
                                       1 if graph else 0, err.ctypes.data_as(C.POINTER(C.c_float)), C.byref(nf)))
         return err[:nchk * batch].reshape(nchk, batch), nf.value
 

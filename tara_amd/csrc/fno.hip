@@ -920,17 +920,22 @@ int rollout_step(tara_fno* f, int nb, const float* in, float* out, bool graph)
 
 }  // namespace
 
-extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps, int every, int nchk,
+extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps, int every, int nens,
                                 tara_ens* const* checkers, int solver_steps, float tol, int use_graph,
                                 float* check_err, int* fallbacks)
 {
-    TARA_CHECK(f && traj && batch >= 1 && batch <= f->maxb && nsteps >= 1 && every >= 1 && nchk == nsteps / every &&
-                   (nchk == 0 || (checkers && solver_steps >= 0)),
-               TARA_ERR_ARG, "fno rollout: bad argument");
-    for (int j = 0; j < nchk; ++j)
-        for (int b = 0; b < batch; ++b)
-            TARA_CHECK(checkers[j] && ens_grid_of(checkers[j], b) == f->n, TARA_ERR_ARG,
-                       "fno rollout: checker %d needs %d runs of grid %d", j, batch, f->n);
+    const int nchk = every >= 1 ? nsteps / every : 0;
+    // one ensemble per check (started as soon as its input exists), or one
+    // ensemble holding every check (all advanced together after the rollout)
+    const bool batched = nchk > 1 && nens == 1;
+    TARA_CHECK(f && traj && batch >= 1 && batch <= f->maxb && nsteps >= 1 && every >= 1 &&
+                   (nens == nchk || batched) && (nchk == 0 || (checkers && solver_steps >= 0)),
+               TARA_ERR_ARG, "fno rollout: bad argument (one checker per check, or one for all)");
+    for (int j = 0; j < nens; ++j) {
+        const int runs = batched ? nchk * batch : batch;
+        TARA_CHECK(checkers[j] && ens_grid_of(checkers[j], runs - 1) == f->n, TARA_ERR_ARG,
+                   "fno rollout: checker %d needs %d runs of grid %d", j, runs, f->n);
+    }
     const size_t np = f->np, per = (size_t)batch * np;
     int rc;
     if (nchk > 0) {
@@ -947,7 +952,7 @@ extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps,
             f->ev_done.push_back(b);
         }
         // capture the checkers' step graphs before any of them is in flight
-        for (int j = 0; j < nchk; ++j)
+        for (int j = 0; j < nens; ++j)
             if ((rc = tara_ens_advance(checkers[j], 0))) return rc;
     }
     const bool graph = use_graph != 0;
@@ -957,7 +962,7 @@ extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps,
     while (s < nsteps) {
         int first_chk = nchk;  // checks launched in this pass: first_chk .. nchk-1
         for (int t = s; t < nsteps; ++t) {
-            if ((t + 1) % every == 0 && (t + 1) / every <= nchk) {
+            if (!batched && (t + 1) % every == 0 && (t + 1) / every <= nchk) {
                 // step t+1 is checked: the solver advances traj[t] by solver_steps
                 const int j = (t + 1) / every - 1;
                 first_chk = std::min(first_chk, j);
@@ -974,6 +979,27 @@ extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps,
             }
             if ((rc = rollout_step(f, batch, traj + (size_t)t * per, traj + (size_t)(t + 1) * per, graph)))
                 return rc;
+        }
+        if (batched) {
+            // every check whose step was (re)computed in this pass, in one ensemble advance
+            first_chk = s / every;
+            if (first_chk < nchk) {
+                tara_ens* e = checkers[0];
+                hipStream_t es = ens_stream(e);
+                TARA_HIP(hipEventRecord(f->ev_in[0], f->st));
+                TARA_HIP(hipStreamWaitEvent(es, f->ev_in[0], 0));
+                for (int j = first_chk; j < nchk; ++j)
+                    for (int b = 0; b < batch; ++b)
+                        if ((rc = tara_ens_set_omega(e, j * batch + b,
+                                                     traj + (size_t)((j + 1) * every - 1) * per + b * np)))
+                            return rc;
+                if ((rc = tara_ens_advance(e, solver_steps))) return rc;
+                for (int j = first_chk; j < nchk; ++j)
+                    for (int b = 0; b < batch; ++b)
+                        if ((rc = tara_ens_get_omega(e, j * batch + b, f->chk.as<float>() + (size_t)j * per + b * np)))
+                            return rc;
+                for (int j = first_chk; j < nchk; ++j) TARA_HIP(hipEventRecord(f->ev_done[j], es));
+            }
         }
         if (first_chk == nchk) break;
         for (int j = first_chk; j < nchk; ++j) TARA_HIP(hipStreamWaitEvent(f->st, f->ev_done[j], 0));

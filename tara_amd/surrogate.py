@@ -91,10 +91,11 @@ def one_step_error(net, snaps, batch=16):
     return float(np.mean(np.concatenate(e_net))), float(np.mean(np.concatenate(e_id)))
 
 
-def rollout(net, ref, nu, dt, steps, every=10, tol=0.05, graph=True, reps=1):
+def rollout(net, ref, nu, dt, steps, every=10, tol=0.05, graph=True, reps=1, batched=True):
     """Checked surrogate rollouts of each trajectory in ref (device float32
     [nsteps + 1][R][n][n], FP64 reference states every `steps` solver steps),
-    one trajectory at a time.
+    one trajectory at a time.  batched: the checks run as one ensemble after
+    the rollout (else one ensemble per check, started as inputs appear).
 
     Returns a dict with, per trajectory: the relative L2 error of the final
     state against the FP64 reference, the largest error over all steps, the
@@ -103,7 +104,10 @@ def rollout(net, ref, nu, dt, steps, every=10, tol=0.05, graph=True, reps=1):
     s1, nr, n, _ = ref.shape
     nsteps = s1 - 1
     nchk = nsteps // every
-    checkers = [Ensemble([n], [nu], dt, SCHEME_RK4) for _ in range(nchk)]
+    if nchk > 1 and batched:
+        checkers = [Ensemble([n] * nchk, [nu] * nchk, dt, SCHEME_RK4)]
+    else:
+        checkers = [Ensemble([n], [nu], dt, SCHEME_RK4) for _ in range(nchk)]
     traj = torch.empty((s1, 1, n, n), dtype=torch.float32, device="cuda")
     out = {"final": [], "max": [], "check_err": [], "fallbacks": [], "ms": []}
     try:

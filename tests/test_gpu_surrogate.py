@@ -100,3 +100,27 @@ def test_failed_checks_hand_steps_to_the_solver(data, trained):
         assert float(torch.linalg.norm(traj[t, 0] - ref) / torch.linalg.norm(ref)) < 1e-6, j
     for c in checkers:
         c.close()
+
+
+def test_batched_checker_matches_separate_checkers(data, trained):
+    """One ensemble holding every check gives the same errors, take-overs and
+    trajectory as one ensemble per check."""
+    _, held = data
+    net, _ = trained
+    nsteps, every = 20, 5
+    outs = []
+    for batched in (False, True):
+        if batched:
+            checkers = [T.Ensemble([N] * (nsteps // every), [NU] * (nsteps // every), DT, T.SCHEME_RK4)]
+        else:
+            checkers = [T.Ensemble([N], [NU], DT, T.SCHEME_RK4) for _ in range(nsteps // every)]
+        traj = torch.empty((nsteps + 1, 1, N, N), device="cuda")
+        traj[0, 0].copy_(held[0, 1])
+        err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=0.0)
+        outs.append((err, nf, traj.cpu().numpy()))
+        for c in checkers:
+            c.close()
+    (e0, n0, t0), (e1, n1, t1) = outs
+    assert n0 == n1 == nsteps // every
+    assert np.allclose(e0, e1, rtol=1e-6, atol=0)
+    assert np.allclose(t0, t1, rtol=0, atol=1e-6 * np.abs(t0).max())
