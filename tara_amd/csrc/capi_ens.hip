@@ -178,8 +178,11 @@ extern "C" int tara_ens_create(int nruns, const int* n, const double* nu, double
         Bucket& b = *bp;
         const int cnt = (int)b.runs.size();
         if ((rc = csh_plan_init(b.plan, b.n, b.n, cnt))) return rc;
+        // per-run slots are bstride apart (padded); the host-side staging
+        // buffer holds exactly one reference-layout spectrum (nh x ny)
         const size_t bytes = b.plan.bstride * sizeof(double2);
-        if ((rc = b.w.alloc(cnt * bytes)) || (rc = b.aux.alloc(cnt * bytes)) || (rc = b.stage.alloc(bytes)) ||
+        const size_t spec_bytes = (size_t)b.plan.nh * b.plan.ny * sizeof(double2);
+        if ((rc = b.w.alloc(cnt * bytes)) || (rc = b.aux.alloc(cnt * bytes)) || (rc = b.stage.alloc(spec_bytes)) ||
             (rc = b.nus.alloc(cnt * sizeof(double))))
             return rc;
         if (scheme == TARA_SCHEME_RK4 && (rc = b.next.alloc(cnt * bytes))) return rc;

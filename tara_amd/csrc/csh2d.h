@@ -27,12 +27,36 @@ struct EventTimer {
     ~EventTimer();
 };
 
+// Shape of the inverse y pass (k_csh_yinv) for column length ny: whether two
+// columns share a workgroup interleaved across its lanes (ny = 4096, 512
+// threads per column), and how many columns a workgroup holds (otherwise
+// contiguous, ny/8 threads each, 256 threads at least).
+__host__ __device__ constexpr bool csh_yinv_interleaved(int ny, bool screened)
+{
+    return !screened && ny == 4096;
+}
+__host__ __device__ constexpr int csh_yinv_cols(int ny, bool screened)
+{
+    const int t = ny / (ny < 8 ? ny : 8);
+    return csh_yinv_interleaved(ny, screened) ? 2 : (t >= 256 ? 1 : 256 / t);
+}
+// Layout of the intermediates G0..G3 (see blkg), a function of nh = nx/2 + 1
+// only, so that the x pass (templated on nx) has it at compile time: log2 of
+// the rows per group, and columns per group.  Nx = 4096 (the interleaved
+// inverse y pass at Ny = 4096): 2-row groups of 2050 columns, an odd multiple
+// of 64 bytes (2052 columns measured 0.206 against 0.1945 ms for the 4096^2
+// inverse y pass).  Otherwise 4-row groups of nh columns.
+__host__ __device__ constexpr int csh_gshift(int nh) { return nh == 4096 / 2 + 1 ? 1 : 2; }
+__host__ __device__ constexpr int csh_gld(int nh) { return nh == 4096 / 2 + 1 ? nh + 1 : nh; }
+
 // Plan + scratch of one (nx, ny) grid: tables and the intermediate fields
 // G0..G3 (after the inverse y pass) and H (after x), all ky-major (nh x ny).
 struct CshPlan {
     int nx = 0, ny = 0, nh = 0;
     int batch = 1;                   // independent runs of this grid (blockIdx.y), fields packed
-    size_t bstride = 0;              // per run: nh * ny complex elements
+    int gld = 0;                     // G0..G3 columns per row group (csh_gld)
+    int gsh = 2;                     // G0..G3 log2 rows per group (csh_gshift)
+    size_t bstride = 0;              // per run: gld * ny complex elements
     DevBuf g[4], h;
     DevBuf kx, ky, imin;
     const double2* tw_y = nullptr;

@@ -51,14 +51,20 @@ __device__ __forceinline__ size_t blk8(int i, int j, int nh)
     return ((size_t)(j >> 3) * nh + i) * 8 + (j & 7);
 }
 
-// Offset of element (i, j) in the intermediates G0..G3: 4-row groups
-// interleaved per kx.  The x pass (2 rows per workgroup at Nx = 4096) then uses
-// half of every 64-byte run it touches and the y pass writes 64-byte runs; at
-// 4096^2 this measured faster than 8-row (yinv 0.20 ms, x 0.39 ms), 2-row
-// (0.25, 0.31) and plain row-major (0.34, 0.31) groups (yinv 0.215, x 0.33).
-__device__ __forceinline__ size_t blkg(int i, int j, int nh)
+// Offset of element (i, j) in the intermediates G0..G3: groups of 2^gs rows
+// interleaved per kx, ld columns per group (CshPlan::gsh, CshPlan::gld).
+// Ny = 4096, 2-row groups: a wave of the x pass reading two rows reads
+// contiguous 32-byte pairs over consecutive kx, and the inverse y pass writes
+// its two interleaved columns' pairs together (YiShape); 0.195 ms inverse y +
+// 0.276 ms x at 4096^2, against 0.186 + 0.311 ms with 4-row groups.  Smaller
+// Ny, 4-row groups and contiguous columns: at 1024^2 0.0201 ms inverse y +
+// 0.0186 ms x, against 0.0236 + 0.0179 ms with 2-row groups and four
+// interleaved columns.  (With one column per inverse-y workgroup at 4096^2,
+// 4-row groups had measured faster than 8-row (0.20, 0.39 ms), 2-row (0.25,
+// 0.31) and plain row-major (0.34, 0.31).)
+__device__ __forceinline__ size_t blkg(int i, int j, int ld, int gs)
 {
-    return ((size_t)(j >> 2) * nh + i) * 4 + (j & 3);
+    return ((size_t)(j >> gs) * ld + i) * (1 << gs) + (j & ((1 << gs) - 1));
 }
 
 // derive2csh's ky: j-1 for j <= Ny/2, j-1-Ny above (exact integers, checked on the host).
@@ -69,14 +75,35 @@ __device__ __forceinline__ double csh_ky(int j)
 }
 
 // --------------------------------------------------------------------- pass 1
+// Shape of the inverse y pass.  At Ny = 4096 (512 threads per column) two
+// columns share a workgroup with their lanes interleaved (column = lane parity):
+// a wave's stores then cover both columns of each 2-row group of G0..G3
+// together (64 B), where one column per workgroup wrote 32-byte pieces.  The
+// second column's LDS buffer is offset by 64 B so the interleaved lanes use
+// other banks.  Other lengths, and the screened form (psi kept live, more
+// registers), keep the contiguous shape of the y passes.
+template <int NY, bool SCR>
+struct YiShape {
+    static constexpr int T = RF<NY>::T;
+    static constexpr bool IL = csh_yinv_interleaved(NY, SCR);
+    static constexpr int CB = csh_yinv_cols(NY, SCR);
+    static_assert(IL || CB == YShape<NY>::CB, "contiguous shape differs from the y passes'");
+    static constexpr int NT = CB * T;
+    static constexpr int LDSC = RF<NY>::LDS + (IL ? 4 : 0);  // per-column exchange buffer
+    // Ny <= 512: at least 56 KB of LDS per workgroup, which holds the pass to
+    // two 256-thread workgroups per CU.  Its registers would allow three, and
+    // at three the ensemble step (128-512 grids) measured 6.67 against 6.42 ms.
+    static constexpr int LDS_ELEMS = (!IL && NY <= 512 && CB * LDSC < 56 * 1024 / 16) ? 56 * 1024 / 16 : CB * LDSC;
+};
+
 // The inverse y pass of one kx column i given omega_hat (wv, at rf_first_idx
 // positions): forms the four spectra of derive2csh.f95:29-71 (i kx omega,
 // i ky omega, i ky psi, -i kx psi; psi = omega/k^2, psi(0,0) = 0), or of
-// derive2css.f95:22-52 when screened, runs the backward y FFTs and writes
-// G0..G3.  x is the column's LDS exchange buffer.
-template <int NY>
+// derive2css.f95:22-52 when SCR (screened), runs the backward y FFTs and writes
+// G0..G3 (row-group stride ld).  x is the column's LDS exchange buffer.
+template <int NY, bool SCR, int GS, bool OPQ>
 __device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i, bool live, int t, double2* x,
-                                                const double2* tw, double lam, int screened, int nh,
+                                                const double2* tw, double lam, int ld,
                                                 double2* __restrict__ g0, double2* __restrict__ g1,
                                                 double2* __restrict__ g2, double2* __restrict__ g3)
 {
@@ -84,8 +111,8 @@ __device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i,
     const double kxv = (double)i;  // derive2csh's kx = i-1 exactly (checked on the host)
     // screened: psi = omega/(k^2 + lambda) with psi(0,0) = 0, omega_d = k^2 psi;
     // the derivatives below act on omega_d, psi is kept in ps.
-    double2 ps[E];
-    if (screened) {
+    double2 ps[SCR ? E : 1];
+    if constexpr (SCR) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int j = rf_first_idx<NY>(t, e);
@@ -100,38 +127,46 @@ __device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i,
 #pragma unroll 1
     for (int f = 0; f < 4; ++f) {
         double2 v[E];
+        // OPQ: the slot index is made opaque per transform so that the index
+        // arithmetic of the transform and the stores is recomputed here rather
+        // than hoisted out of the loop and held in registers (178 -> 117 VGPRs
+        // at Ny = 4096, which lets a 1024-thread workgroup run unspilled).
+        const int tt = OPQ ? rf_opaque(t) : t;
         if (f == 2) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const double kyv = csh_ky<NY>(rf_first_idx<NY>(t, e));
-                const double k2 = kxv * kxv + kyv * kyv;
-                wv[e] = screened ? ps[e]
-                                 : (k2 != 0.0) ? make_double2(wv[e].x / k2, wv[e].y / k2) : make_double2(0.0, 0.0);
+                if constexpr (SCR) {
+                    wv[e] = ps[e];
+                } else {
+                    const double kyv = csh_ky<NY>(rf_first_idx<NY>(tt, e));
+                    const double k2 = kxv * kxv + kyv * kyv;
+                    wv[e] = (k2 != 0.0) ? make_double2(wv[e].x / k2, wv[e].y / k2) : make_double2(0.0, 0.0);
+                }
             }
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const double kyv = csh_ky<NY>(rf_first_idx<NY>(t, e));
+            const double kyv = csh_ky<NY>(rf_first_idx<NY>(tt, e));
             const double kk = (f == 0 || f == 3) ? kxv : kyv;
             const double2 z = wv[e];
             v[e] = (f == 3) ? make_double2(kk * z.y, -(kk * z.x))     // -i kx psi
                             : make_double2(-(kk * z.y), kk * z.x);    // i k (omega | psi)
         }
-        rf_fft<NY, +1>(v, t, x, tw);
+        rf_fft<NY, +1>(v, tt, x, tw);
         // G0 = ux (f 2), G1 = uy (f 3), G2 = d omega/dx (f 0), G3 = d omega/dy (f 1)
         double2* o = f == 0 ? g2 : f == 1 ? g3 : f == 2 ? g0 : g1;
         if (live) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) o[blkg(i, rf_final_idx<NY>(t, e), nh)] = v[e];
+            for (int e = 0; e < E; ++e) o[blkg(i, rf_final_idx<NY>(tt, e), ld, GS)] = v[e];
         }
     }
 }
 
-template <int NY>
-__global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, int nh,
-                                                              double2* __restrict__ g0, double2* __restrict__ g1,
-                                                              double2* __restrict__ g2, double2* __restrict__ g3,
-                                                              const double2* tw, double lam, int screened, size_t bs)
+template <int NY, bool SCR, int GS>
+__global__ __launch_bounds__((YiShape<NY, SCR>::NT), (YiShape<NY, SCR>::IL ? 4 : 1)) void k_csh_yinv(const double2* w, int nh,
+                                                                    double2* __restrict__ g0, double2* __restrict__ g1,
+                                                                    double2* __restrict__ g2, double2* __restrict__ g3,
+                                                                    const double2* tw, double lam, int ld, size_t bs)
 {
     // batched runs: blockIdx.y selects the run, every field is strided by bs
     const size_t rb = blockIdx.y * bs;
@@ -140,17 +175,18 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
     g1 += rb;
     g2 += rb;
     g3 += rb;
-    using S = YShape<NY>;
+    using S = YiShape<NY, SCR>;
     constexpr int E = RF<NY>::E, T = S::T;
-    __shared__ double2 lds[S::CB * RF<NY>::LDS];
-    const int c = threadIdx.x / T, t = threadIdx.x % T;
+    __shared__ double2 lds[S::LDS_ELEMS];
+    const int c = S::IL ? (int)threadIdx.x % S::CB : (int)threadIdx.x / T;
+    const int t = S::IL ? (int)threadIdx.x / S::CB : (int)threadIdx.x % T;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * S::CB + c;
     const bool live = i < nh;
     const size_t col = (size_t)(live ? i : 0) * NY;
     double2 wv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) wv[e] = live ? w[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
-    csh_yinv_column<NY>(wv, i, live, t, lds + c * RF<NY>::LDS, tw, lam, screened, nh, g0, g1, g2, g3);
+    csh_yinv_column<NY, SCR, GS, S::IL>(wv, i, live, t, lds + c * S::LDSC, tw, lam, ld, g0, g1, g2, g3);
 }
 
 // --------------------------------------------------------------------- pass 2
@@ -160,15 +196,16 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yinv(const double2* w, i
 // x_a * x_b is Re(y) Im(y).  The imaginary parts of X[0] and X[M] are dropped,
 // as FFTW's c2r does.  Registers end with y at rf_final_idx<NX> positions.
 template <int NX, int EM, bool SPLIT>
-__device__ __forceinline__ void x_c2r_pair(const double2* ga, const double2* gb, int ny, int j, int t,
+__device__ __forceinline__ void x_c2r_pair(const double2* ga, const double2* gb, int j, int t,
                                            double2* xl, const double2* tw, double2 (&v)[RF<NX, EM>::E])
 {
     constexpr int M = NX / 2;
+    constexpr int LD = csh_gld(M + 1), GS = csh_gshift(M + 1);
 #pragma unroll
     for (int e = 0; e < RF<NX, EM>::E; ++e) {
         const int k = rf_first_idx<NX, EM>(t, e);
         const int kk = k <= M ? k : NX - k;
-        double2 a = ga[blkg(kk, j, M + 1)], b = gb[blkg(kk, j, M + 1)];
+        double2 a = ga[blkg(kk, j, LD, GS)], b = gb[blkg(kk, j, LD, GS)];
         if (kk == 0 || kk == M) {
             a.y = 0.0;
             b.y = 0.0;
@@ -201,7 +238,7 @@ struct XShape {
 
 template <int NX, int EM, int RB, bool SPLIT>
 __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
-    const double2* g0, const double2* g1, const double2* g2, const double2* g3, int ny, double scale,
+    const double2* g0, const double2* g1, const double2* g2, const double2* g3, double scale,
     double2* __restrict__ h, const double2* tw_n, const double2* tw_m, const double2* rctw, size_t bs)
 {
     const size_t rb = blockIdx.y * bs;  // batched runs
@@ -219,10 +256,10 @@ __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
     double2* xl = lds + r * S::LDSR;
     double nl[E];
     double2 v[E];
-    x_c2r_pair<NX, EM, SPLIT>(g0, g2, ny, j, t, xl, tw_n, v);   // ux + i d omega/dx
+    x_c2r_pair<NX, EM, SPLIT>(g0, g2, j, t, xl, tw_n, v);   // ux + i d omega/dx
 #pragma unroll
     for (int e = 0; e < E; ++e) nl[e] = v[e].x * v[e].y;
-    x_c2r_pair<NX, EM, SPLIT>(g1, g3, ny, j, t, xl, tw_n, v);   // uy + i d omega/dy
+    x_c2r_pair<NX, EM, SPLIT>(g1, g3, j, t, xl, tw_n, v);   // uy + i d omega/dy
 #pragma unroll
     for (int e = 0; e < E; ++e) nl[e] = (nl[e] + v[e].x * v[e].y) * scale;
     // forward x transform of the real product row: pack z[m] = nl[2m] + i nl[2m+1]
@@ -352,14 +389,22 @@ __global__ void k_ab2(const double2* w, double2* __restrict__ wn, const double2*
 }
 
 // ------------------------------------------------------------------ dispatch
+template <int NY, bool SCR, int GS>
+static int launch_yinv_t(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream_t st)
+{
+    using S = YiShape<NY, SCR>;
+    hipLaunchKernelGGL((k_csh_yinv<NY, SCR, GS>), dim3((p.nh + S::CB - 1) / S::CB, p.batch), dim3(S::NT), 0, st, w,
+                       p.nh, p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(),
+                       p.tw_y, ph.lambda, p.gld, p.bstride);
+    return TARA_OK;
+}
+
 template <int NY>
 static int launch_yinv(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream_t st)
 {
-    using S = YShape<NY>;
-    hipLaunchKernelGGL(k_csh_yinv<NY>, dim3((p.nh + S::CB - 1) / S::CB, p.batch), dim3(S::NT), 0, st, w, p.nh,
-                       p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(),
-                       p.tw_y, ph.lambda, ph.screened, p.bstride);
-    return TARA_OK;
+    if (p.gsh == 1)
+        return ph.screened ? launch_yinv_t<NY, true, 1>(p, w, ph, st) : launch_yinv_t<NY, false, 1>(p, w, ph, st);
+    return ph.screened ? launch_yinv_t<NY, true, 2>(p, w, ph, st) : launch_yinv_t<NY, false, 2>(p, w, ph, st);
 }
 
 // x-pass shapes the autotuner chooses among: {EM, RB, SPLIT}.  Shape 0 is
@@ -390,7 +435,7 @@ static int launch_x_shape(const CshPlan& p, hipStream_t st)
         const double2* twm = EM == 16 ? p.tw_x16 : p.tw_x;
         hipLaunchKernelGGL((k_csh_x<NX, EM, RB, SPLIT>), dim3(p.ny / RB, p.batch), dim3(S::NT), 0, st,
                            p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(),
-                           p.ny, scale, p.h.as<double2>(), twn, twm, p.rctw, p.bstride);
+                           scale, p.h.as<double2>(), twn, twm, p.rctw, p.bstride);
         return TARA_OK;
     }
 }
@@ -480,7 +525,9 @@ int csh_plan_init(CshPlan& p, int nx, int ny, int batch)
     p.nx = nx;
     p.ny = ny;
     p.nh = nx / 2 + 1;
-    const size_t ns = (size_t)p.nh * ny;
+    p.gld = csh_gld(p.nh);
+    p.gsh = csh_gshift(p.nh);
+    const size_t ns = (size_t)p.gld * ny;  // G0..G3 hold gld columns per row group
     p.batch = batch;
     p.bstride = ns;
     int rc;

@@ -35,6 +35,15 @@ struct RF {
 
 __device__ __forceinline__ int rf_pad(int n) { return n + (n >> 3); }
 
+// v, through an empty asm statement the compiler cannot see into: index
+// arithmetic derived from the result is recomputed where it is used instead of
+// being hoisted out of a loop of transforms and held in registers.
+__device__ __forceinline__ int rf_opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 template <bool WAVE>
 __device__ __forceinline__ void rf_sync()
 {
