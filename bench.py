@@ -30,19 +30,19 @@ HBM_PEAK_GBS = 8000.0
 # --no-cpu-baseline --no-ensemble --no-fno), each scaled by what the counter reads for a device copy
 # of known size (tara_amd/calib/copy_probe: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
 MEASURED_TRAFFIC = {
-    ("k_csh_yinv", 4096): 6.8311e8,
-    ("k_csh_yfwd", 4096): 6.7417e8,      # mean over the four RK4 stage variants
+    ("k_csh_yinv", 4096): 7.1623e8,
+    ("k_csh_yfwd", 4096): 6.7424e8,      # mean over the four RK4 stage variants
     # k_csh_x per x-pass shape (the autotuner's choice, see tara_csh_tune)
-    ("k_csh_x", 4096, 0): 7.1846e8,
-    ("k_csh_x", 4096, 1): 7.0652e8,
-    ("k_csh_x", 4096, 2): 6.7951e8,
-    ("k_csh_x", 4096, 3): 2.7119e9,
-    ("k_csh_x", 4096, 4): 7.0797e8,
-    ("k_csh_x", 4096, 5): 6.8332e8,
-    ("k_mhd_col_inverse_zy", 256): 3.9930e9,   # two launches (z, then y), 1.9965e9 each
-    ("k_mhd_x", 256): 2.9068e9,
-    ("k_mhd_col_forward", 256): 3.9928e9,
-    ("k_mhd_zfwd", 256): 6.9324e9,
+    ("k_csh_x", 4096, 0): 9.4684e8,
+    ("k_csh_x", 4096, 1): 7.1346e8,
+    ("k_csh_x", 4096, 2): 6.7887e8,
+    ("k_csh_x", 4096, 3): 2.7156e9,
+    ("k_csh_x", 4096, 4): 7.1700e8,
+    ("k_csh_x", 4096, 5): 6.9173e8,
+    ("k_mhd_col_inverse_zy", 256): 3.9932e9,   # two launches (z, then y), 1.9966e9 each
+    ("k_mhd_x", 256): 2.9069e9,
+    ("k_mhd_col_forward", 256): 3.9931e9,
+    ("k_mhd_zfwd", 256): 6.9318e9,
 }
 METRIC = "grid-pt steps/s FP64 (2D 4096², 3D 256³ MHD); FNO sim-time/s at ≤5% L2 err"
 
