@@ -158,8 +158,9 @@ struct ZArgs {
 
 // The z pass holds 14 transforms' worth of accumulators; at 4 elements per
 // thread and a bound of 4 waves per SIMD (128 VGPRs, two 512-thread workgroups
-// per CU) it ran 1.48 ms at 256^3, against 1.85 ms at 8 elements per thread
-// (256 VGPRs, one 256-thread workgroup per CU) and 1.57 ms unbounded.
+// per CU) it ran 1.48 ms at 256^3 (1.44 ms with the AB2 form's opaque slot
+// index below), against 1.85 ms at 8 elements per thread (256 VGPRs, one
+// 256-thread workgroup per CU) and 1.57 ms unbounded.
 constexpr int kZfwdEM = 4;
 template <int NZ, int MODE, int EM>
 __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, const double2* tw)
@@ -187,9 +188,14 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
     }
     auto zfft = [&](int f, double2 (&v)[E]) {
         const double2* src = a.b + (size_t)f * a.fs + base;
+        // AB2: the slot index is made opaque per transform so that the index
+        // arithmetic is redone here rather than held across all 14 transforms
+        // (spill 84 -> 28 bytes per lane at 128 VGPRs); the RHS form spills
+        // more that way and keeps it.
+        const int tt = MODE == MHD_AB2 ? rf_opaque(t) : t;
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = src[(size_t)rf_first_idx<NZ, EM>(t, e) * es];
-        rf_fft<NZ, -1, EM>(v, t, xl, tw);
+        for (int e = 0; e < E; ++e) v[e] = src[(size_t)rf_first_idx<NZ, EM>(tt, e) * es];
+        rf_fft<NZ, -1, EM>(v, tt, xl, tw);
     };
     // i k z = (-k z.y, k z.x)
     auto ik = [](double kk, double2 z) { return make_double2(-(kk * z.y), kk * z.x); };
