@@ -30,21 +30,39 @@ HBM_PEAK_GBS = 8000.0
 # --no-cpu-baseline --no-ensemble --no-fno), each scaled by what the counter reads for a device copy
 # of known size (tara_amd/calib/copy_probe: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
 MEASURED_TRAFFIC = {
-    ("k_csh_yinv", 4096): 7.1623e8,
+    ("k_csh_yinv", 4096): 7.1266e8,
     ("k_csh_yfwd", 4096): 6.7424e8,      # mean over the four RK4 stage variants
+    ("k_csh_yfwd_inv", 4096): 1.2985e9,  # mean over the four RK4 stage variants
     # k_csh_x per x-pass shape (the autotuner's choice, see tara_csh_tune)
-    ("k_csh_x", 4096, 0): 9.4684e8,
-    ("k_csh_x", 4096, 1): 7.1346e8,
-    ("k_csh_x", 4096, 2): 6.7887e8,
-    ("k_csh_x", 4096, 3): 2.7156e9,
-    ("k_csh_x", 4096, 4): 7.1700e8,
-    ("k_csh_x", 4096, 5): 6.9173e8,
+    ("k_csh_x", 4096, 0): 9.4956e8,
+    ("k_csh_x", 4096, 1): 7.1194e8,
+    ("k_csh_x", 4096, 2): 6.7919e8,
+    ("k_csh_x", 4096, 3): 2.7163e9,
+    ("k_csh_x", 4096, 4): 7.1465e8,
+    ("k_csh_x", 4096, 5): 6.9267e8,
     ("k_mhd_col_inverse_zy", 256): 3.9932e9,   # two launches (z, then y), 1.9966e9 each
     ("k_mhd_x", 256): 2.9069e9,
     ("k_mhd_col_forward", 256): 3.9931e9,
-    ("k_mhd_zfwd", 256): 6.9318e9,
+    ("k_mhd_zfwd", 256): 6.6727e9,
 }
 METRIC = "grid-pt steps/s FP64 (2D 4096², 3D 256³ MHD); FNO sim-time/s at ≤5% L2 err"
+
+
+def model_2d(n, nsteps):
+    """Algorithmic bytes of the 2D RK4 path at n^2 for one advance call of
+    nsteps steps: per timing class (mean per launch), class names, and bytes
+    per step.  S = one half spectrum.  The inverse y pass reads 1 field and
+    writes 4; the x pass reads 4 and writes 1; the forward y pass of RK4 stages
+    1..4 moves 4, 6, 6 and 4 fields.  From Ny = 1024 each forward y pass is fused
+    with the next stage's inverse y pass (timing class 2, + 4 S written, no
+    read), except the call's last, and class 0 is the call's one stand-alone
+    inverse y pass: (36 nsteps - 4) S in class 2, 56 S per step.  Unfused:
+    5 S per launch in every class, 60 S per step."""
+    S = 16.0 * (n // 2 + 1) * n
+    if n >= 1024:
+        return ({0: 5 * S, 1: 5 * S, 2: (36 * nsteps - 4) * S / (4 * nsteps)},
+                {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd_inv"}, 56 * S)
+    return {0: 5 * S, 1: 5 * S, 2: 5 * S}, {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd"}, 60 * S
 
 
 def cpu_baseline(n):
@@ -87,13 +105,13 @@ def bench_2d(n, steps, warmup, nu=1e-4, dt=2e-4):
     s.sync()
     counts, kms = s.kernel_stats(3)
     s.close()
-    S = 16.0 * (n // 2 + 1) * n
-    names = {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd"}
+    model, names, step_bytes = model_2d(n, steps)
     kernels = {names[c]: {"avg_ms": round(kms[c] / counts[c], 4), "launches": counts[c],
-                          "GBps": round(5 * S / (kms[c] / counts[c] * 1e-3) / 1e9, 1)} for c in range(3) if counts[c]}
+                          "GBps": round(model[c] / (kms[c] / counts[c] * 1e-3) / 1e9, 1)}
+               for c in range(3) if counts[c]}
     return {"workload": f"2D incompressible hydro, RK4, {n}^2, FP64", "value": n * n / (ms * 1e-3),
             "unit": "grid-pt steps/s", "ms_per_step": ms, "steps": reps, "xpass_shape": xshape, "kernels": kernels,
-            "step_model_bytes": 60 * S, "step_model_GBps": round(60 * S / (ms * 1e-3) / 1e9, 1)}
+            "step_model_bytes": step_bytes, "step_model_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def bench_mhd(steps, warmup, n=256):
@@ -332,12 +350,7 @@ def main():
     s.close()
     finite = bool(abs(w).max() < 1e300)
 
-    S = 16.0 * (n // 2 + 1) * n                      # one half spectrum, bytes
-    # algorithmic bytes per launch: inverse y reads 1 field and writes 4; the x
-    # pass reads 4 and writes 1; the forward y pass (RK4 stages 1..4) moves
-    # 4, 6, 6, 4 fields, 5 on average.  60 S per RK4 step.
-    model = {0: 5 * S, 1: 5 * S, 2: 5 * S}
-    names = {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd"}
+    model, names, step_bytes = model_2d(n, a.steps)  # bytes per launch and per step (see model_2d)
     avg = {c: ms[c] / counts[c] for c in range(3) if counts[c]}
     dom = max(avg, key=lambda c: ms[c])
     achieved = model[dom] / (avg[dom] * 1e-3) / 1e9
@@ -369,8 +382,8 @@ def main():
                      "algorithmic_bytes_per_launch": model[dom]},
         "kernels": kernels,
         "xpass_shape": xshape,
-        "step_model_bytes": 60 * S,
-        "step_model_GBps": round(60 * S / (ms_per_step * 1e-3) / 1e9, 1),
+        "step_model_bytes": step_bytes,
+        "step_model_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "state_finite": finite,
         "cpu_baseline": cpu,
         "secondary_2d_1024": sec_1024,

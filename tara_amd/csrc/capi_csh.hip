@@ -305,6 +305,7 @@ extern "C" int tara_csh_advance(tara_csh* c, int nsteps)
     double2* aux = c->aux.as<double2>();
     double2* nx = c->next.as<double2>();
     EventTimer* tm = c->tm.get();
+    if (!c->ph.screened) return csh_advance(c->plan, c->scheme, nsteps, c->ph, c->dt, w, aux, nx, c->st, tm);
     for (int s = 0; s < nsteps; ++s) {
         int rc;
         if (c->scheme == TARA_SCHEME_AB2) {

@@ -55,24 +55,35 @@ void drop_graph(tara_ens* e)
 }
 
 // One step of every bucket, forked onto the bucket streams and joined back.
+// Each forward y pass is fused with the following inverse y pass, the last
+// stage's too: a step expects G0..G3 to hold the inverse y pass of the state
+// (enqueue_head) and leaves them holding that of the new state, so that the
+// captured step can be replayed back to back.
 int enqueue_step(tara_ens* e)
 {
     TARA_HIP(hipEventRecord(e->fork, e->st));
     for (auto& bp : e->b) {
         Bucket& b = *bp;
         TARA_HIP(hipStreamWaitEvent(b.st, e->fork, 0));
-        double2* w = b.w.as<double2>();
-        double2* aux = b.aux.as<double2>();
-        double2* nx = b.next.as<double2>();
-        int rc;
-        if (e->scheme == TARA_SCHEME_AB2) {
-            if ((rc = csh_rhs_stage(b.plan, w, CSH_AB2, b.ph, e->dt, w, aux, nullptr, b.st, nullptr))) return rc;
-        } else {
-            if ((rc = csh_rhs_stage(b.plan, w, CSH_RK4_1, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
-            if ((rc = csh_rhs_stage(b.plan, nx, CSH_RK4_2, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
-            if ((rc = csh_rhs_stage(b.plan, nx, CSH_RK4_3, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
-            if ((rc = csh_rhs_stage(b.plan, nx, CSH_RK4_4, b.ph, e->dt, w, aux, nx, b.st, nullptr))) return rc;
-        }
+        int rc = csh_fused_step(b.plan, e->scheme, b.ph, e->dt, b.w.as<double2>(), b.aux.as<double2>(),
+                                b.next.as<double2>(), false, b.st, nullptr);
+        if (rc) return rc;
+        TARA_HIP(hipEventRecord(b.done, b.st));
+        TARA_HIP(hipStreamWaitEvent(e->st, b.done, 0));
+    }
+    return TARA_OK;
+}
+
+// The inverse y pass of every bucket's state, once per advance call (the state
+// may have been set or uploaded since the last step).
+int enqueue_head(tara_ens* e)
+{
+    TARA_HIP(hipEventRecord(e->fork, e->st));
+    for (auto& bp : e->b) {
+        Bucket& b = *bp;
+        TARA_HIP(hipStreamWaitEvent(b.st, e->fork, 0));
+        int rc = csh_yinv_pass(b.plan, b.w.as<double2>(), b.ph, b.st);
+        if (rc) return rc;
         TARA_HIP(hipEventRecord(b.done, b.st));
         TARA_HIP(hipStreamWaitEvent(e->st, b.done, 0));
     }
@@ -288,7 +299,9 @@ extern "C" int tara_ens_advance(tara_ens* e, int nsteps)
 {
     TARA_CHECK(e && nsteps >= 0, TARA_ERR_ARG, "bad argument");
     int rc;
+    if (nsteps == 0) return TARA_OK;
     if (e->use_graph && !e->exec && (rc = capture(e))) return rc;
+    if ((rc = enqueue_head(e))) return rc;
     for (int s = 0; s < nsteps; ++s) {
         if (e->use_graph) {
             TARA_HIP(hipGraphLaunch(e->exec, e->st));

@@ -86,6 +86,21 @@ int csh_plan_init(CshPlan& p, int nx, int ny, int batch = 1);
 // One right-hand-side evaluation on stage input ws followed by the fused
 // update selected by mode (see k_csh_yfwd).  Timing classes: 0 inverse y,
 // 1 x pass, 2 forward y + update.
+// nsteps AB2 or RK4 steps (scheme: TARA_SCHEME_*) of derive2csh on the resident
+// state w, with each forward y pass fused with the inverse y pass that follows
+// it (k_csh_yfwd_inv): one stand-alone inverse y pass per call.  Timing classes
+// as csh_rhs_stage (class 2 holds the fused passes).
+int csh_advance(const CshPlan& p, int scheme, int nsteps, const CshPhys& ph, double dt, double2* w,
+                double2* aux, double2* next, hipStream_t st, EventTimer* tm);
+// The parts of csh_advance: the stand-alone inverse y pass of w into G0..G3,
+// and one step that expects G0..G3 to hold it (x pass, fused forward + inverse
+// y pass per stage; plain_last: the last stage's forward y pass alone, else
+// G0..G3 are left holding the inverse y pass of the new state).  For Ny < 1024
+// the passes are not fused: csh_yinv_pass does nothing and the step runs
+// csh_rhs_stage per stage.
+int csh_yinv_pass(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream_t st);
+int csh_fused_step(const CshPlan& p, int scheme, const CshPhys& ph, double dt, double2* w, double2* aux,
+                   double2* next, bool plain_last, hipStream_t st, EventTimer* tm);
 int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& ph, double dt, double2* base,
                   double2* aux, double2* next, hipStream_t st, EventTimer* tm);
 // x-pass shapes: the default for nx, how many exist, whether shape c runs on

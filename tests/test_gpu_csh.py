@@ -178,6 +178,33 @@ def test_rk4_4096_one_step_consistency():
     assert rel_l2(got, ref) < 1e-14
 
 
+@pytest.mark.parametrize("n,scheme", [(4096, T.SCHEME_RK4), (4096, T.SCHEME_AB2), (1024, T.SCHEME_RK4),
+                                      (256, T.SCHEME_AB2), (2048, T.SCHEME_RK4)])
+def test_advance_one_call_matches_single_steps(n, scheme):
+    """Within one advance call each forward y pass is fused with the next
+    stage's inverse y pass, across step boundaries too; a call of one step ends
+    with a plain forward y pass and the next call starts with a stand-alone
+    inverse y pass.  Three steps in one call must equal three one-step calls
+    (the same arithmetic on the same values, so equal to rounding)."""
+    nu, dt = 1e-4, 2e-4
+    a = T.CshSolver(n, n, nu, dt, scheme)
+    b = T.CshSolver(n, n, nu, dt, scheme)
+    try:
+        a.ic_random(seed=3, k0=6.0)
+        b.ic_random(seed=3, k0=6.0)
+        a.advance(3)
+        for _ in range(3):
+            b.advance(1)
+        wa, wb = a.download(), b.download()
+    finally:
+        a.close()
+        b.close()
+    assert np.all(np.isfinite(wa))
+    err = rel_l2(wa, wb)
+    print(f"  n={n} scheme={scheme}: one call vs single steps rel L2 {err:.2e}", flush=True)
+    assert err < 1e-14
+
+
 # ------------------------------------------------------------------ x-pass shapes
 @pytest.mark.parametrize("nx,ny", [(64, 64), (256, 512), (1024, 1024), (4096, 256)])
 def test_every_xshape_matches_oracle(nx, ny):
