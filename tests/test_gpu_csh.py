@@ -1,7 +1,7 @@
 """GPU parity of the 2D path, through the C ABI (tara_amd -> libtara_hip.so).
 
-Bar: relative L2 <= 1e-10 against the reference trajectory (north-star
-tolerance, FP64); single calls and transforms are held to 1e-13 / 1e-14, which
+Bar: relative L2 <= 1e-10 against the reference trajectory (the required
+FP64 tolerance); single calls and transforms are held to 1e-13 / 1e-14, which
 is the rounding level of two different FFT factorisations.
 """
 import numpy as np
