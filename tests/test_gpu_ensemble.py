@@ -56,6 +56,48 @@ def test_graph_replays_match_eager():
     ee.close()
 
 
+def test_fused_buckets_follow_uploads_and_replays():
+    """Buckets of Ny >= 1024 run each forward y pass fused with the next inverse
+    y pass, and every advance call starts with a stand-alone inverse y pass of
+    the state.  A run whose state is uploaded between two advance calls must
+    continue from the new state; every run must match the same sequence done
+    alone, and graph replays must equal eager launches."""
+    print("fused ensemble buckets", flush=True)
+    n = [1024, 256, 1024, 2048]
+    nu = [1e-3, 1e-3, 1e-4, 5e-4]
+    dt = 2e-4
+    w0 = [random_phase_vorticity(m, seed=40 + r, k0=4.0 + r) for r, m in enumerate(n)]
+    w_new = random_phase_vorticity(1024, seed=99, k0=6.0)
+    eg = T.Ensemble(n, nu, dt, T.SCHEME_RK4)
+    ee = T.Ensemble(n, nu, dt, T.SCHEME_RK4)
+    ee.use_graph(False)
+    try:
+        for e in (eg, ee):
+            for r, w in enumerate(w0):
+                e.upload(r, w)
+            e.advance(3)
+            e.upload(2, w_new)
+            e.advance(2)
+        got = [eg.download(r) for r in range(len(n))]
+        for r in range(len(n)):
+            assert np.array_equal(got[r], ee.download(r)), r
+    finally:
+        eg.close()
+        ee.close()
+    for r, m in enumerate(n):
+        s = T.CshSolver(m, m, nu[r], dt, T.SCHEME_RK4)
+        s.upload(w0[r])
+        s.advance(3)
+        if r == 2:
+            s.upload(w_new)
+        s.advance(2)
+        want = s.download()
+        s.close()
+        err = rel_l2(got[r], want)
+        print(f"  run {r} ({m}^2): ensemble vs single run rel L2 {err:.2e}", flush=True)
+        assert err < 1e-14, r
+
+
 @pytest.mark.parametrize("tag", ["asrun", "intent"])
 def test_ensemble_ab2_config1_vs_reference(golden, tag):
     """Two copies of the config-1 run plus a differently sized run in one
