@@ -158,9 +158,11 @@ struct ZArgs {
 
 // The z pass holds 14 transforms' worth of accumulators; at 4 elements per
 // thread and a bound of 4 waves per SIMD (128 VGPRs, two 512-thread workgroups
-// per CU) it ran 1.48 ms at 256^3 (1.44 ms with the AB2 form's opaque slot
-// index below), against 1.85 ms at 8 elements per thread (256 VGPRs, one
-// 256-thread workgroup per CU) and 1.57 ms unbounded.
+// per CU) it ran 1.48 ms at 256^3, against 1.85 ms at 8 elements per thread
+// (256 VGPRs, one 256-thread workgroup per CU) and 1.57 ms unbounded.  The AB2
+// form's opaque slot index took it to 1.44 ms (spill 84 -> 28 bytes per lane),
+// forming the density term from the momentum loads of the updates to 1.38 ms
+// (110 VGPRs, no spill, 3 fields fewer read).
 constexpr int kZfwdEM = 4;
 template <int NZ, int MODE, int EM>
 __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, const double2* tw)
@@ -190,8 +192,8 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
         const double2* src = a.b + (size_t)f * a.fs + base;
         // AB2: the slot index is made opaque per transform so that the index
         // arithmetic is redone here rather than held across all 14 transforms
-        // (spill 84 -> 28 bytes per lane at 128 VGPRs); the RHS form spills
-        // more that way and keeps it.
+        // (110 VGPRs against 124); the RHS form spills more that way and keeps
+        // it.
         const int tt = MODE == MHD_AB2 ? rf_opaque(t) : t;
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = src[(size_t)rf_first_idx<NZ, EM>(tt, e) * es];
@@ -199,6 +201,11 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
     };
     // i k z = (-k z.y, k z.x)
     auto ik = [](double kk, double2 z) { return make_double2(-(kk * z.y), kk * z.x); };
+    // AB2: dens accumulates the density term -(i kx rho ux + i ky rho uy +
+    // i kz rho uz) from the momentum state that the updates of fields 1..3 load
+    // anyway, so the momentum spectra are read once (not again for the density
+    // term before the 14 transforms); density is finished after field 3.
+    double2 dens[E];
     auto finish = [&](int fld, const double2 (&acc)[E]) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -207,19 +214,24 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
                 a.out[fld][q[e]] = f;
             } else {
                 const double2 fo = a.out[fld][q[e]], w = a.st[fld][q[e]];
+                if (fld == 1) dens[e] = ik(kxv, w);
+                if (fld == 2) { const double2 s2 = ik(kyv, w); dens[e] = make_double2(dens[e].x + s2.x, dens[e].y + s2.y); }
+                if (fld == 3) { const double2 s3 = ik(kzv[e], w); dens[e] = make_double2(dens[e].x + s3.x, dens[e].y + s3.y); }
                 a.st[fld][q[e]] = make_double2(w.x + (1.5 * f.x - 0.5 * fo.x) * a.dt, w.y + (1.5 * f.y - 0.5 * fo.y) * a.dt);
                 a.out[fld][q[e]] = f;
             }
         }
     };
     double2 v[E], acc[E], acc2[E];
-    // density: -(i kx rho ux + i ky rho uy + i kz rho uz) on the state spectra
+    if (MODE == MHD_RHS) {
+        // density: -(i kx rho ux + i ky rho uy + i kz rho uz) on the state spectra
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const double2 s1 = ik(kxv, a.st[1][q[e]]), s2 = ik(kyv, a.st[2][q[e]]), s3 = ik(kzv[e], a.st[3][q[e]]);
-        acc[e] = make_double2(-((s1.x + s2.x) + s3.x), -((s1.y + s2.y) + s3.y));
+        for (int e = 0; e < E; ++e) {
+            const double2 s1 = ik(kxv, a.st[1][q[e]]), s2 = ik(kyv, a.st[2][q[e]]), s3 = ik(kzv[e], a.st[3][q[e]]);
+            acc[e] = make_double2(-((s1.x + s2.x) + s3.x), -((s1.y + s2.y) + s3.y));
+        }
+        finish(0, acc);
     }
-    finish(0, acc);
     // rho ux: -(i k . (MX1, MX2, MX3)) - mu k^2 ux; MX2 also feeds rho uy (Mom_y_1 == Mom_x_2)
     zfft(0, v);
 #pragma unroll
@@ -264,6 +276,11 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
 #pragma unroll
     for (int e = 0; e < E; ++e) { const double2 d = ik(kzv[e], v[e]); acc[e].x -= d.x; acc[e].y -= d.y; }
     finish(3, acc);
+    if (MODE == MHD_AB2) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[e] = make_double2(-dens[e].x, -dens[e].y);
+        finish(0, acc);
+    }
     // induction: dBx = i ky MG1 + i kz MG2, dBy = -i kx MG1 + i kz MG3,
     // dBz = -i kx MG2 - i ky MG3, each minus eta k^2 B (src/derive3com.f95:1030-1037)
     zfft(11, v);  // MG1
