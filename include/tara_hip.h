@@ -77,7 +77,10 @@ int tara_csh_destroy(tara_csh* ctx);
 /* Kernel timing: when on, each launch is bracketed by HIP events on the
  * context's stream.  tara_csh_kernel_stats returns, per kernel class
  * (0 = inverse y pass, 1 = x pass, 2 = forward y pass + update), the number of
- * timed launches and their summed duration in milliseconds, then resets. */
+ * timed launches and their summed duration in milliseconds, then resets.  For
+ * the derive2csh solver at Ny >= 1024, class 2 launches (all but each advance
+ * call's last) also run the next stage's inverse y pass, and class 0 holds one
+ * launch per advance call. */
 int tara_csh_set_timing(tara_csh* ctx, int on);
 int tara_csh_kernel_stats(tara_csh* ctx, int nclass, long* counts, double* ms);
 /* Autotuner (no counterpart in the reference, whose FFTW planner runs in
