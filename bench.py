@@ -43,7 +43,7 @@ MEASURED_TRAFFIC = {
     ("k_mhd_col_inverse_zy", 256): 3.9932e9,   # two launches (z, then y), 1.9966e9 each
     ("k_mhd_x", 256): 2.9069e9,
     ("k_mhd_col_forward", 256): 3.9931e9,
-    ("k_mhd_zfwd", 256): 6.6727e9,
+    ("k_mhd_zfwd", 256): 6.1173e9,
 }
 METRIC = "grid-pt steps/s FP64 (2D 4096², 3D 256³ MHD); FNO sim-time/s at ≤5% L2 err"
 
