@@ -206,14 +206,16 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
     // anyway, so the momentum spectra are read once (not again for the density
     // term before the 14 transforms); density is finished after field 3.
     double2 dens[E];
-    auto finish = [&](int fld, const double2 (&acc)[E]) {
+    // wpre: the state of field fld at this lane's positions when the caller
+    // has already loaded it (AB2), else null.
+    auto finish = [&](int fld, const double2 (&acc)[E], const double2* wpre = nullptr) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const double2 f = keep[e] ? acc[e] : make_double2(0.0, 0.0);
             if (MODE == MHD_RHS) {
                 a.out[fld][q[e]] = f;
             } else {
-                const double2 fo = a.out[fld][q[e]], w = a.st[fld][q[e]];
+                const double2 fo = a.out[fld][q[e]], w = wpre ? wpre[e] : a.st[fld][q[e]];
                 if (fld == 1) dens[e] = ik(kxv, w);
                 if (fld == 2) { const double2 s2 = ik(kyv, w); dens[e] = make_double2(dens[e].x + s2.x, dens[e].y + s2.y); }
                 if (fld == 3) { const double2 s3 = ik(kzv[e], w); dens[e] = make_double2(dens[e].x + s3.x, dens[e].y + s3.y); }
@@ -291,25 +293,27 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
         acc2[e] = make_double2(-g.x, -g.y);
     }
     zfft(12, v);  // MG2
-    double2 acc3[E];
+    double2 acc3[E], bs[E];  // bs: the B component just loaded, reused by its update
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const double2 d = ik(kzv[e], v[e]), g = ik(kxv, v[e]);
-        const double2 bx = a.st[4][q[e]];
-        acc[e] = make_double2((acc[e].x + d.x) - a.eta * k2[e] * bx.x, (acc[e].y + d.y) - a.eta * k2[e] * bx.y);
+        bs[e] = a.st[4][q[e]];
+        acc[e] = make_double2((acc[e].x + d.x) - a.eta * k2[e] * bs[e].x, (acc[e].y + d.y) - a.eta * k2[e] * bs[e].y);
         acc3[e] = make_double2(-g.x, -g.y);
     }
-    finish(4, acc);
+    finish(4, acc, bs);
     zfft(13, v);  // MG3
+    double2 bz[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const double2 d = ik(kzv[e], v[e]), g = ik(kyv, v[e]);
-        const double2 by = a.st[5][q[e]], bz = a.st[6][q[e]];
-        acc2[e] = make_double2((acc2[e].x + d.x) - a.eta * k2[e] * by.x, (acc2[e].y + d.y) - a.eta * k2[e] * by.y);
-        acc3[e] = make_double2((acc3[e].x - g.x) - a.eta * k2[e] * bz.x, (acc3[e].y - g.y) - a.eta * k2[e] * bz.y);
+        bs[e] = a.st[5][q[e]];
+        bz[e] = a.st[6][q[e]];
+        acc2[e] = make_double2((acc2[e].x + d.x) - a.eta * k2[e] * bs[e].x, (acc2[e].y + d.y) - a.eta * k2[e] * bs[e].y);
+        acc3[e] = make_double2((acc3[e].x - g.x) - a.eta * k2[e] * bz[e].x, (acc3[e].y - g.y) - a.eta * k2[e] * bz[e].y);
     }
-    finish(5, acc2);
-    finish(6, acc3);
+    finish(5, acc2, bs);
+    finish(6, acc3, bz);
 }
 
 // ------------------------------------------------------------------ dispatch
