@@ -137,7 +137,8 @@ def test_rk4_resident_vs_reference(golden):
     s.close()
 
 
-@pytest.mark.parametrize("nx,ny,steps", [(256, 256, 20), (1024, 1024, 3), (512, 2048, 2)])
+@pytest.mark.parametrize("nx,ny,steps", [(256, 256, 20), (1024, 1024, 3), (512, 2048, 2), (4096, 1024, 1),
+                                           (1024, 4096, 1)])
 def test_rk4_resident_vs_oracle(nx, ny, steps):
     print(f"rk4 {nx}x{ny}", flush=True)
     x = np.arange(nx) * 2 * PI / nx

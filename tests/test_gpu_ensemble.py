@@ -56,7 +56,8 @@ def test_graph_replays_match_eager():
     ee.close()
 
 
-def test_fused_buckets_follow_uploads_and_replays():
+@pytest.mark.parametrize("scheme", [T.SCHEME_RK4, T.SCHEME_AB2])
+def test_fused_buckets_follow_uploads_and_replays(scheme):
     """Buckets of Ny >= 1024 run each forward y pass fused with the next inverse
     y pass, and every advance call starts with a stand-alone inverse y pass of
     the state.  A run whose state is uploaded between two advance calls must
@@ -68,8 +69,8 @@ def test_fused_buckets_follow_uploads_and_replays():
     dt = 2e-4
     w0 = [random_phase_vorticity(m, seed=40 + r, k0=4.0 + r) for r, m in enumerate(n)]
     w_new = random_phase_vorticity(1024, seed=99, k0=6.0)
-    eg = T.Ensemble(n, nu, dt, T.SCHEME_RK4)
-    ee = T.Ensemble(n, nu, dt, T.SCHEME_RK4)
+    eg = T.Ensemble(n, nu, dt, scheme)
+    ee = T.Ensemble(n, nu, dt, scheme)
     ee.use_graph(False)
     try:
         for e in (eg, ee):
@@ -85,7 +86,7 @@ def test_fused_buckets_follow_uploads_and_replays():
         eg.close()
         ee.close()
     for r, m in enumerate(n):
-        s = T.CshSolver(m, m, nu[r], dt, T.SCHEME_RK4)
+        s = T.CshSolver(m, m, nu[r], dt, scheme)
         s.upload(w0[r])
         s.advance(3)
         if r == 2:
