@@ -50,7 +50,8 @@ __host__ __device__ constexpr int csh_gshift(int nh) { return nh == 4096 / 2 + 1
 __host__ __device__ constexpr int csh_gld(int nh) { return nh == 4096 / 2 + 1 ? nh + 1 : nh; }
 
 // Plan + scratch of one (nx, ny) grid: tables and the intermediate fields
-// G0..G3 (after the inverse y pass) and H (after x), all ky-major (nh x ny).
+// G0..G3 (after the inverse y pass) and H (after x), in the row-group layouts
+// of csh2d.hip (blkg with gld columns and 2^gsh rows per group; blk8).
 struct CshPlan {
     int nx = 0, ny = 0, nh = 0;
     int batch = 1;                   // independent runs of this grid (blockIdx.y), fields packed
@@ -83,9 +84,6 @@ struct CshPhys {
 // batch: number of independent runs of this grid the launches cover; every
 // per-run field (state, G0..G3, H) is packed run after run, bstride apart.
 int csh_plan_init(CshPlan& p, int nx, int ny, int batch = 1);
-// One right-hand-side evaluation on stage input ws followed by the fused
-// update selected by mode (see k_csh_yfwd).  Timing classes: 0 inverse y,
-// 1 x pass, 2 forward y + update.
 // nsteps AB2 or RK4 steps (scheme: TARA_SCHEME_*) of derive2csh on the resident
 // state w, with each forward y pass fused with the inverse y pass that follows
 // it (k_csh_yfwd_inv): one stand-alone inverse y pass per call.  Timing classes
@@ -101,6 +99,9 @@ int csh_advance(const CshPlan& p, int scheme, int nsteps, const CshPhys& ph, dou
 int csh_yinv_pass(const CshPlan& p, const double2* w, const CshPhys& ph, hipStream_t st);
 int csh_fused_step(const CshPlan& p, int scheme, const CshPhys& ph, double dt, double2* w, double2* aux,
                    double2* next, bool plain_last, hipStream_t st, EventTimer* tm);
+// One right-hand-side evaluation on stage input ws followed by the fused
+// update selected by mode (see k_csh_yfwd).  Timing classes: 0 inverse y,
+// 1 x pass, 2 forward y + update.
 int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& ph, double dt, double2* base,
                   double2* aux, double2* next, hipStream_t st, EventTimer* tm);
 // x-pass shapes: the default for nx, how many exist, whether shape c runs on
