@@ -26,6 +26,7 @@ extern "C" {
 #define TARA_ERR_ALLOC (-3)
 #define TARA_ERR_UNSUPPORTED (-4)
 #define TARA_ERR_STATE (-5)
+#define TARA_ERR_NONFINITE (-6) /* a resident state became NaN or Inf (see the *_sync calls) */
 
 #define TARA_SCHEME_AB2 1
 #define TARA_SCHEME_RK4 2
@@ -72,6 +73,10 @@ int tara_csh_create(int nx, int ny, double nu, double dt, int scheme, tara_csh**
 int tara_csh_upload(tara_csh* ctx, const double* omegak);   /* zeroes the AB2 history */
 int tara_csh_download(tara_csh* ctx, double* omegak);
 int tara_csh_advance(tara_csh* ctx, int nsteps);             /* asynchronous */
+/* Waits for the context's work.  Guard: the last stage of every step flags a
+ * new state value that is NaN or Inf; sync (and download, which still copies
+ * the state out) then return TARA_ERR_NONFINITE until a new state is set
+ * (upload or an initial condition).  The reference has no such check. */
 int tara_csh_sync(tara_csh* ctx);
 int tara_csh_destroy(tara_csh* ctx);
 /* Kernel timing: when on, each launch is bracketed by HIP events on the
@@ -128,10 +133,19 @@ int tara_ens_create(int nruns, const int* n, const double* nu, double dt, int sc
 int tara_ens_upload(tara_ens* ctx, int run, const double* omegak);
 int tara_ens_download(tara_ens* ctx, int run, double* omegak);
 int tara_ens_advance(tara_ens* ctx, int nsteps);
+/* sync returns TARA_ERR_NONFINITE when any run's state became NaN or Inf (the
+ * guard of tara_csh_sync, per run); tara_ens_nonfinite gives the number of such
+ * runs and, if flags (nruns ints) is not NULL, 1 for each of them.  A run's flag
+ * clears when its state is set again (upload, ic_random, set_omega); the other
+ * runs are not affected by one run's failure. */
 int tara_ens_sync(tara_ens* ctx);
+int tara_ens_nonfinite(tara_ens* ctx, int* flags);
 int tara_ens_use_graph(tara_ens* ctx, int on);
 int tara_ens_tune(tara_ens* ctx, int reps);
 int tara_ens_destroy(tara_ens* ctx);
+/* The captured step graph: total nodes and kernel nodes (0 before the first
+ * advance call; tara_ens_advance(ctx, 0) captures it without stepping). */
+int tara_ens_graph_nodes(tara_ens* ctx, long* nodes, long* kernels);
 /* Device-side data paths of one run, ordered on the ensemble's stream:
  * ic_random sets the random-phase vorticity of tara_csh_ic_random (synchronous);
  * set_omega / get_omega move real-space vorticity (device float, n*n, x fastest)
@@ -279,6 +293,9 @@ int tara_mhd_create(int nx, int ny, int nz, const tara_mhd_params* p, tara_mhd**
 int tara_mhd_upload(tara_mhd* ctx, const double* const state[8]);
 int tara_mhd_download(tara_mhd* ctx, double* const state[8]);
 int tara_mhd_advance(tara_mhd* ctx, int nsteps);
+/* Waits for the context's work; TARA_ERR_NONFINITE (also from download, which
+ * still copies the state out) when an AB2 step produced a NaN or Inf state
+ * value, until a new state is set (upload or an initial condition). */
 int tara_mhd_sync(tara_mhd* ctx);
 int tara_mhd_destroy(tara_mhd* ctx);
 /* The reference's initial state, src/main3com.f95:290-324 (ABC flow): uniform
@@ -289,6 +306,15 @@ int tara_mhd_destroy(tara_mhd* ctx);
  * rho0 = 1, u0 = 0.1, b0 = 0.1 sqrt(mu_0), p0 = 1, a = b = cc = kf = 1 (:228-268). */
 int tara_mhd_ic_abc(tara_mhd* ctx, double rho0, double u0, double b0, double p0, double a, double b, double cc,
                     double kf);
+/* Random-phase initial state generated on the device (history zeroed): rho = 1,
+ * solenoidal u and B with shell spectrum ~ k^4 exp(-2 (k/k0)^2), scaled to
+ * 1/2 <u^2> = e_kin and 1/2 <B^2> = e_mag, rho u = u, E = 1 + u^2/2 + B^2/2.
+ * Each mode's three phases come from Philox-4x32-10 with key {seed, 1} (u) or
+ * {seed, 2} (B), counters {m, 0, 0, 0} and {m, 1, 0, 0}, m = i + Nh (j + Ny l);
+ * the kx = 0 plane is Hermitian, k = 0 and the Nyquist planes are zero, and the
+ * raw vector is projected with its own wavevector.  No counterpart in the
+ * reference (its 3D initial state is the ABC flow of tara_mhd_ic_abc). */
+int tara_mhd_ic_random(tara_mhd* ctx, unsigned seed, double k0, double e_kin, double e_mag);
 /* Kernel timing as for tara_csh: classes 0 = inverse z+y passes, 1 = x pass,
  * 2 = forward y pass, 3 = forward z pass + assembly + update. */
 int tara_mhd_set_timing(tara_mhd* ctx, int on);
@@ -331,16 +357,23 @@ int tara_fno_rel_err(tara_fno* f, int batch, const float* a, const float* ref, f
  * [nsteps+1][batch][n][n], traj[0] set by the caller; step t+1 = network(step t).
  * Every `every`-th step t+1 = (j+1)*every (j < nchk = nsteps / every) is
  * re-checked: the solver (the checker ensembles' viscosity, dt and scheme, grid
- * n) advances traj[t] by solver_steps steps.  nens = nchk: ensemble j (batch
+ * n) advances traj[t] by solver_steps steps and the check's error is the
+ * relative L2 distance of step t+1 from that.  nens = nchk: ensemble j (batch
  * runs) runs check j on its own streams as soon as traj[t] exists, concurrently
  * with the rollout; nens = 1: one ensemble of nchk*batch runs (run j*batch + b)
- * advances every check together once the rollout's steps are done.  When a
- * check's relative L2 error exceeds tol, the solver's field replaces that
- * sample's step (j+1)*every and the rollout resumes from there.  check_err
- * (host, [nchk][batch]) receives each check's error, fallbacks the number of
- * take-overs.  use_graph replays a captured network step.  Synchronous. */
+ * advances every check together once the rollout's steps are done.
+ * anchor (batch runs, same solver; needed when nchk > 0) holds the last state
+ * known to be the FP64 solution, traj[0] to begin with.  When a check's error
+ * exceeds tol for any sample, the FP64 solver takes the trajectory over: the
+ * anchor advances from its step to the failing step T = (j+1)*every, its fields
+ * replace traj[T] for every sample, T becomes the anchor's step and the rollout
+ * resumes from there.  So every state after a take-over is the FP64 solution
+ * from traj[0] (with tol = 0 every check fails and each traj[(j+1)*every] is
+ * the plain solver's).  check_err (host, [nchk][batch]) receives each check's
+ * error, fallbacks the number of take-overs.  use_graph replays a captured
+ * network step.  Synchronous. */
 int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps, int every, int nens, tara_ens* const* checkers,
-                     int solver_steps, float tol, int use_graph, float* check_err, int* fallbacks);
+                     tara_ens* anchor, int solver_steps, float tol, int use_graph, float* check_err, int* fallbacks);
 int tara_fno_destroy(tara_fno* f);
 
 /* ------------------------------------------------------ snapshots and data

@@ -59,6 +59,7 @@ def lib():
         _orc.orc_derive2com.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _orc.orc_derive3com.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _orc.orc_ab3com.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 4
+        _orc.orc_ic_random3d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint] + [C.c_double] * 3 + [C.c_void_p]
     return _orc
 
 
@@ -77,6 +78,7 @@ def ref():
         _ref.ref_ic2csh.argtypes = [C.c_int, C.c_int] + [C.c_double] * 5 + [_dp, _dp]
         _ref.ref_ab2csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
         _ref.ref_rk4csh_steps.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp]
+        _ref.ref_mhd_ab2_steps.argtypes = [C.c_int] * 5 + [_dp, _dp]
     return _ref
 
 
@@ -192,6 +194,13 @@ def ab3com(state, f_old, f_new, nx, dt):
     ny, nz = state[0].shape[1:]
     out = [np.zeros_like(state[0], order="F") for _ in range(8)]
     lib().orc_ab3com(nx, ny, nz, dt, _ptr_array(state), _ptr_array(out), _ptr_array(f_old), _ptr_array(f_new))
+    return out
+
+
+def ic_random3d(n, seed=1, k0=3.0, e_kin=1.0, e_mag=0.5):
+    """Eight (Nh, N, N) spectra of the random-phase 3D state (orc_ic_random3d)."""
+    out = [zeros_spec(n, n, n) for _ in range(8)]
+    lib().orc_ic_random3d(n, n, n, seed, k0, e_kin, e_mag, _ptr_array(out))
     return out
 
 
@@ -385,21 +394,60 @@ def ref_comm():
     return _Comm.in_dll(ref(), "comm_")
 
 
-def ref_derive3com(state, nx, lx, ly, lz, cs=10.0, threads=1):
-    """Calls the reference derive3com (diffusion hard-wired to 1/450, Q9)."""
-    state = [np.array(s, dtype=np.complex128, order="F") for s in state]
-    nh, ny, nz = state[0].shape
+def _set_comm3(nx, ny, nz, lx, ly, lz, cs, threads, mu=None, eta=None, dt=0.0):
     cm = ref_comm()
     cm.thread_num = threads
     cm.Lx, cm.Ly, cm.Lz = lx, ly, lz
     cm.CS = cs
     cm.kf = 1.0
     cm.dx, cm.dy, cm.dz = lx / nx, ly / ny, lz / nz
+    cm.dt = dt
+    if mu is not None:
+        cm.mu, cm.eta = mu, eta
+    return cm
+
+
+def ref_derive3com(state, nx, lx, ly, lz, cs=10.0, threads=1, mu=None, eta=None):
+    """Calls the reference derive3com (diffusion hard-wired to 1/450, Q9) or,
+    with mu and eta given, derive3com_mueta: the same routine with its 1/450
+    replaced by COMMON's mu (momentum) and eta (induction), built by
+    oracle/Makefile from a patched copy of src/derive3com.f95."""
+    state = [np.array(s, dtype=np.complex128, order="F") for s in state]
+    nh, ny, nz = state[0].shape
+    _set_comm3(nx, ny, nz, lx, ly, lz, cs, threads, mu, eta)
     old = [np.zeros_like(state[0], order="F") for _ in range(8)]
     new = [np.zeros_like(state[0], order="F") for _ in range(8)]
     args = [_i(nx), _i(ny), _i(nz), _i(nh), _d(3.14159265358979323846), _d(0.0)]
     args += [_ptr(s) for s in state]
     for o, n in zip(old, new):
         args += [_ptr(o), _ptr(n)]
-    ref().derive3com_(*args)
+    (ref().derive3com_ if mu is None else ref().derive3com_mueta_)(*args)
     return new
+
+
+def ref_ab3com(state, f_old, f_new, nx, dt):
+    """Calls the reference ab3com (src/ab3com.f95:6-66; dt from COMMON)."""
+    state, f_old, f_new = ([np.array(s, dtype=np.complex128, order="F") for s in x] for x in (state, f_old, f_new))
+    nh, ny, nz = state[0].shape
+    ref_comm().dt = dt
+    new = [np.zeros_like(state[0], order="F") for _ in range(8)]
+    args = [_i(nx), _i(ny), _i(nz), _i(nh), _d(3.14159265358979323846), _d(0.0)]
+    args += [_ptr(s) for s in state] + [_ptr(s) for s in new]
+    for o, n in zip(f_old, f_new):
+        args += [_ptr(o), _ptr(n)]
+    ref().ab3com_(*args)
+    return new
+
+
+def ref_mhd_ab2_steps(state, nx, lx, ly, lz, mu, eta, dt, nsteps, cs=10.0, threads=1, hist=None):
+    """The reference's 3D AB2 loop (src/main3com.f95:536-617) over
+    derive3com_mueta + ab3com, history starting at zero unless given.
+    Returns (state, hist) after nsteps steps."""
+    nh, ny, nz = state[0].shape
+    st = np.stack([np.asarray(s, dtype=np.complex128) for s in state], axis=-1)
+    st = np.asfortranarray(st)
+    hs = np.zeros_like(st, order="F") if hist is None else np.asfortranarray(
+        np.stack([np.asarray(s, dtype=np.complex128) for s in hist], axis=-1))
+    _set_comm3(nx, ny, nz, lx, ly, lz, cs, threads, mu, eta, dt)
+    ref().ref_mhd_ab2_steps(nx, ny, nz, nsteps, 1, _ptr(st), _ptr(hs))
+    return ([np.asfortranarray(st[..., f]) for f in range(8)], [np.asfortranarray(hs[..., f]) for f in range(8)])

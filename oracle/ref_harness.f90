@@ -100,4 +100,52 @@ contains
     deallocate(w, k1, k2, k3, k4, dum)
   end subroutine
 
+  ! AB2 loop of src/main3com.f95:536-617: derive3com, ab3com, then the history
+  ! and state rotation.  patched /= 0 calls derive3com_mueta, the reference's
+  ! derive3com with its hard-wired 1/450 diffusion replaced by COMMON's mu and
+  ! eta (oracle/Makefile); 0 calls the unmodified derive3com.  The physics
+  ! parameters (Lx, Ly, Lz, CS, mu, eta, dt, thread_num) are read from COMMON
+  ! /comm/, which the caller fills.  st(:,:,:,f) and hist(:,:,:,f) hold the
+  ! fields in the argument order of derive3com (rho, rho ux, rho uy, rho uz, E,
+  ! Bx, By, Bz); both are advanced in place (hist = the last RHS on return).
+  subroutine ref_mhd_ab2_steps(nx, ny, nz, nsteps, patched, st, hist) bind(C, name="ref_mhd_ab2_steps")
+    integer(c_int), value :: nx, ny, nz, nsteps, patched
+    complex(c_double_complex) :: st(nx/2+1, ny, nz, 8), hist(nx/2+1, ny, nz, 8)
+    integer :: iret, thread_num
+    real(8) :: time_max, Lx, Ly, Lz, dx, dy, dz, spheat, mu, ms, CS, mu_0, eta, dt, kf
+    common/comm/iret,thread_num,time_max,Lx,Ly,Lz,dx,dy,dz,spheat,mu,ms,CS,mu_0,eta,dt,kf
+    complex(8), allocatable :: sn(:,:,:,:), fn(:,:,:,:)
+    integer :: t, nh
+    real(8) :: time
+    nh = nx/2 + 1
+    allocate(sn(nh, ny, nz, 8), fn(nh, ny, nz, 8))
+    time = 0.0d0
+    do t = 1, nsteps
+      if (patched /= 0) then
+        call derive3com_mueta(nx, ny, nz, nh, pi, time, st(:,:,:,1), st(:,:,:,2), st(:,:,:,3), st(:,:,:,4), &
+                              st(:,:,:,5), st(:,:,:,6), st(:,:,:,7), st(:,:,:,8), &
+                              hist(:,:,:,1), fn(:,:,:,1), hist(:,:,:,2), fn(:,:,:,2), hist(:,:,:,3), fn(:,:,:,3), &
+                              hist(:,:,:,4), fn(:,:,:,4), hist(:,:,:,5), fn(:,:,:,5), &
+                              hist(:,:,:,6), fn(:,:,:,6), hist(:,:,:,7), fn(:,:,:,7), hist(:,:,:,8), fn(:,:,:,8))
+      else
+        call derive3com(nx, ny, nz, nh, pi, time, st(:,:,:,1), st(:,:,:,2), st(:,:,:,3), st(:,:,:,4), &
+                        st(:,:,:,5), st(:,:,:,6), st(:,:,:,7), st(:,:,:,8), &
+                        hist(:,:,:,1), fn(:,:,:,1), hist(:,:,:,2), fn(:,:,:,2), hist(:,:,:,3), fn(:,:,:,3), &
+                        hist(:,:,:,4), fn(:,:,:,4), hist(:,:,:,5), fn(:,:,:,5), &
+                        hist(:,:,:,6), fn(:,:,:,6), hist(:,:,:,7), fn(:,:,:,7), hist(:,:,:,8), fn(:,:,:,8))
+      end if
+      call ab3com(nx, ny, nz, nh, pi, time, st(:,:,:,1), st(:,:,:,2), st(:,:,:,3), st(:,:,:,4), &
+                  st(:,:,:,5), st(:,:,:,6), st(:,:,:,7), st(:,:,:,8), &
+                  sn(:,:,:,1), sn(:,:,:,2), sn(:,:,:,3), sn(:,:,:,4), sn(:,:,:,5), sn(:,:,:,6), sn(:,:,:,7), &
+                  sn(:,:,:,8), &
+                  hist(:,:,:,1), fn(:,:,:,1), hist(:,:,:,2), fn(:,:,:,2), hist(:,:,:,3), fn(:,:,:,3), &
+                  hist(:,:,:,4), fn(:,:,:,4), hist(:,:,:,5), fn(:,:,:,5), &
+                  hist(:,:,:,6), fn(:,:,:,6), hist(:,:,:,7), fn(:,:,:,7), hist(:,:,:,8), fn(:,:,:,8))
+      hist = fn
+      st = sn
+      time = time + dt
+    end do
+    deallocate(sn, fn)
+  end subroutine
+
 end module ref_harness

@@ -632,6 +632,103 @@ void orc_ic_random(int nx, int ny, unsigned seed, double k0, double energy, doub
     for (size_t p = 0; p < 2 * (size_t)nh * ny; ++p) wk[p] *= scale;
 }
 
+/* Uniform double in [0, 1) from two Philox words (53 random bits). */
+static double orc_u53(unsigned a, unsigned b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
+}
+
+/* One random-phase solenoidal vector field (three half spectra v[c], each
+ * (Nh,Ny,Nz)) with |v(k)| per component k exp(-(k/k0)^2), i.e. shell
+ * spectrum ~ k^4 exp(-2 (k/k0)^2), scaled to 1/2 <|v|^2> = energy.  Mode
+ * (i,j,l) takes the three phases 2 pi u_c from Philox-4x32-10 with key
+ * {seed, tag} and counters {m, 0, 0, 0}, {m, 1, 0, 0}, m = i + Nh (j + Ny l):
+ * u_0 from words 0-1 of the first draw, u_1 from words 2-3, u_2 from words
+ * 0-1 of the second.  On the kx = 0 plane the mode whose mirror (-ky, -kz)
+ * has the smaller index j + Ny l takes the conjugate of the mirror's draw
+ * (Hermitian).  The raw vector is projected with its own wavevector,
+ * v <- v - k (k . v)/k^2.  k = 0 and the Nyquist planes (i = Nh-1,
+ * j = Ny/2, l = Nz/2) are zero.  No counterpart in the reference, whose 3D
+ * initial state is the deterministic ABC flow (src/main3com.f95:290-324). */
+static void ic_solenoidal(int nx, int ny, int nz, unsigned seed, unsigned tag, double k0, double energy,
+                          double *const v[3])
+{
+    const double pi = 3.14159265358979323846;
+    const int nh = nx / 2 + 1;
+    double s = 0.0;
+    for (int l = 0; l < nz; ++l)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nh; ++i) {
+                const size_t p = (size_t)i + (size_t)nh * ((size_t)j + (size_t)ny * l);
+                for (int c = 0; c < 3; ++c) { RE(v[c], p) = 0.0; IM(v[c], p) = 0.0; }
+                if (i == nh - 1 || j == ny / 2 || l == nz / 2 || (i == 0 && j == 0 && l == 0)) continue;
+                int jc = j, lc = l, conj = 0;
+                if (i == 0) {
+                    const int jm = (ny - j) % ny, lm = (nz - l) % nz;
+                    if (jm + ny * lm < j + ny * l) { jc = jm; lc = lm; conj = 1; }
+                }
+                const double kv[3] = {(double)i, (double)(j < ny / 2 ? j : j - ny), (double)(l < nz / 2 ? l : l - nz)};
+                const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+                const double k = sqrt(k2);
+                const double amp = k * exp(-(k / k0) * (k / k0));
+                const unsigned m = (unsigned)((size_t)i + (size_t)nh * ((size_t)jc + (size_t)ny * lc));
+                unsigned c0[4] = {m, 0, 0, 0}, c1[4] = {m, 1, 0, 0}, key[2] = {seed, tag}, r[4], q[4];
+                orc_philox4x32_10(c0, key, r);
+                orc_philox4x32_10(c1, key, q);
+                const double u[3] = {orc_u53(r[0], r[1]), orc_u53(r[2], r[3]), orc_u53(q[0], q[1])};
+                double re[3], im[3];
+                for (int c = 0; c < 3; ++c) {
+                    const double ph = 2.0 * pi * u[c];
+                    re[c] = amp * cos(ph);
+                    im[c] = conj ? -(amp * sin(ph)) : amp * sin(ph);
+                }
+                const double dr = (kv[0] * re[0] + kv[1] * re[1] + kv[2] * re[2]) / k2;
+                const double di = (kv[0] * im[0] + kv[1] * im[1] + kv[2] * im[2]) / k2;
+                const double w = i == 0 ? 1.0 : 2.0;
+                for (int c = 0; c < 3; ++c) {
+                    RE(v[c], p) = re[c] - kv[c] * dr;
+                    IM(v[c], p) = im[c] - kv[c] * di;
+                    s += w * (RE(v[c], p) * RE(v[c], p) + IM(v[c], p) * IM(v[c], p));
+                }
+            }
+    /* 1/2 <|v|^2> = 1/2 sum over the full spectrum |v_k|^2 / (Nx Ny Nz)^2 */
+    const double nn = (double)nx * ny * nz;
+    const double e = 0.5 * s / (nn * nn);
+    const double scale = e > 0 ? sqrt(energy / e) : 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (size_t p = 0; p < 2 * (size_t)nh * ny * nz; ++p) v[c][p] *= scale;
+}
+
+void orc_ic_random3d(int nx, int ny, int nz, unsigned seed, double k0, double e_kin, double e_mag,
+                     double *const st[8])
+{
+    const int nh = nx / 2 + 1;
+    const size_t ns = (size_t)nh * ny * nz, nr = (size_t)nx * ny * nz;
+    double *u[3] = {st[1], st[2], st[3]}, *b[3] = {st[5], st[6], st[7]};
+    ic_solenoidal(nx, ny, nz, seed, 1u, k0, e_kin, u);
+    ic_solenoidal(nx, ny, nz, seed, 2u, k0, e_mag, b);
+    /* rho = 1: its unnormalised transform is Nx Ny Nz at k = 0; rho u = u */
+    memset(st[0], 0, 2 * ns * sizeof(double));
+    st[0][0] = (double)nr;
+    /* E = P + rho u^2/2 + B^2/2 with P = 1 (as src/main3com.f95:324-325 forms it) */
+    double *re = (double *)malloc(nr * sizeof(double)), *acc = (double *)calloc(nr, sizeof(double));
+    double *tmp = (double *)malloc(2 * ns * sizeof(double));
+    const double inv = 1.0 / (double)nr;
+    for (int f = 0; f < 6; ++f) {
+        memcpy(tmp, f < 3 ? u[f] : b[f - 3], 2 * ns * sizeof(double));
+        orc_c2r_3d(nx, ny, nz, tmp, re);
+        for (size_t q = 0; q < nr; ++q) {
+            const double x = re[q] * inv;
+            acc[q] += x * x;
+        }
+    }
+    for (size_t q = 0; q < nr; ++q) re[q] = 1.0 + 0.5 * acc[q];
+    orc_r2c_3d(nx, ny, nz, re, st[4]);
+    free(re);
+    free(acc);
+    free(tmp);
+}
+
 /* Shear-layer IC of src/main2csh.f95:84-110 (x-fastest real field, then r2c). */
 void orc_ic2csh(int nx, int ny, double lx_in, double ly_in, double w0, double m0,
                 double d_in, double *omega, double *omegak)

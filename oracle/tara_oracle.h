@@ -66,6 +66,12 @@ void orc_derive3com(int nx, int ny, int nz, const orc_mhd_params *p,
 void orc_ab3com(int nx, int ny, int nz, double dt, const double *const state[8],
                 double *const state_new[8], const double *const f_old[8],
                 const double *const f_new[8]);
+/* Random-phase initial state (no reference counterpart): rho = 1, solenoidal
+   u and B with shell spectrum ~ k^4 exp(-2 (k/k0)^2), 1/2 <u^2> = e_kin,
+   1/2 <B^2> = e_mag (Philox-4x32-10 phases, keys {seed, 1} and {seed, 2}),
+   rho u = u, E = 1 + u^2/2 + B^2/2. */
+void orc_ic_random3d(int nx, int ny, int nz, unsigned seed, double k0, double e_kin, double e_mag,
+                     double *const st[8]);
 
 #ifdef __cplusplus
 }

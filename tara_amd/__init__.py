@@ -34,8 +34,15 @@ SCHEME_AB2 = 1
 SCHEME_RK4 = 2
 
 
+ERR_NONFINITE = -6
+
+
 class TaraError(RuntimeError):
-    pass
+    """A nonzero status of the C ABI; .code holds it (e.g. ERR_NONFINITE)."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 class MhdParams(C.Structure):
@@ -103,6 +110,8 @@ def lib():
         L.tara_ens_use_graph.argtypes = [C.c_void_p, C.c_int]
         L.tara_ens_tune.argtypes = [C.c_void_p, C.c_int]
         L.tara_ens_ic_random.argtypes = [C.c_void_p, C.c_int, C.c_uint, C.c_double, C.c_double]
+        L.tara_ens_graph_nodes.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.tara_ens_nonfinite.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.tara_ens_set_omega.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tara_ens_get_omega.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         _fp = C.POINTER(C.c_float)
@@ -116,7 +125,7 @@ def lib():
         L.tara_fno_rel_err.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _fp]
         L.tara_fno_destroy.argtypes = [C.c_void_p]
         L.tara_fno_rollout.argtypes = ([C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
-                                        C.POINTER(C.c_void_p), C.c_int, C.c_float, C.c_int, _fp,
+                                        C.POINTER(C.c_void_p), C.c_void_p, C.c_int, C.c_float, C.c_int, _fp,
                                         C.POINTER(C.c_int)])
         P3, P6 = C.c_void_p * 3, C.c_void_p * 6
         L.tara_derive2coh.argtypes = [C.c_int] * 3 + [C.POINTER(CohParams), P3, P3]
@@ -146,6 +155,7 @@ def lib():
             getattr(L, n).argtypes = [C.c_void_p]
         L.tara_mhd_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.tara_mhd_ic_abc.argtypes = [C.c_void_p] + [C.c_double] * 8
+        L.tara_mhd_ic_random.argtypes = [C.c_void_p, C.c_uint] + [C.c_double] * 3
         L.tara_mhd_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
         _lib = L
     return _lib
@@ -153,7 +163,7 @@ def lib():
 
 def _check(rc):
     if rc != 0:
-        raise TaraError(f"tara error {rc}: {lib().tara_last_error().decode()}")
+        raise TaraError(f"tara error {rc}: {lib().tara_last_error().decode()}", rc)
 
 
 def _p(a):
@@ -393,19 +403,23 @@ class FNO:
                                       out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
-    def rollout(self, batch, traj, nsteps, every=10, checkers=(), solver_steps=100, tol=0.05, graph=True):
+    def rollout(self, batch, traj, nsteps, every=10, checkers=(), solver_steps=100, tol=0.05, graph=True,
+                anchor=None):
         """Surrogate rollout of traj[0] (device float32 [nsteps+1][batch][n][n])
         with FP64 checks every `every` steps by the Ensemble objects in
         `checkers`: nsteps // every of them with `batch` runs each (checks start
         as their inputs appear), or one with (nsteps // every) * batch runs (all
-        checks advanced together after the rollout).  Returns (check errors
+        checks advanced together after the rollout).  `anchor` (an Ensemble of
+        `batch` runs of the same solver) carries the FP64 solution that takes
+        the trajectory over at a failed check.  Returns (check errors
         [nchk][batch], number of solver take-overs)."""
         nens = len(checkers)
         nchk = nsteps // every
         hs = (C.c_void_p * max(nens, 1))(*[c._h for c in checkers])
         err = np.zeros(max(nchk * batch, 1), np.float32)
         nf = C.c_int()
-        _check(lib().tara_fno_rollout(self._h, batch, _dptr(traj), nsteps, every, nens, hs, solver_steps, tol,
+        _check(lib().tara_fno_rollout(self._h, batch, _dptr(traj), nsteps, every, nens, hs,
+                                      anchor._h if anchor is not None else None, solver_steps, tol,
                                       1 if graph else 0, err.ctypes.data_as(C.POINTER(C.c_float)), C.byref(nf)))
         return err[:nchk * batch].reshape(nchk, batch), nf.value
 
@@ -477,6 +491,20 @@ class Ensemble:
         """One run's real-space vorticity into device float32 memory (asynchronous
         until sync())."""
         _check(lib().tara_ens_get_omega(self._h, run, _dptr(out)))
+
+    def nonfinite(self):
+        """Per-run flags (list of bool) of runs whose state became NaN or Inf."""
+        flags = (C.c_int * len(self.n))()
+        rc = lib().tara_ens_nonfinite(self._h, flags)
+        if rc < 0:
+            _check(rc)
+        return [bool(f) for f in flags]
+
+    def graph_nodes(self):
+        """(nodes, kernel nodes) of the captured step graph; (0, 0) before capture."""
+        n, k = C.c_long(), C.c_long()
+        _check(lib().tara_ens_graph_nodes(self._h, C.byref(n), C.byref(k)))
+        return n.value, k.value
 
     def tune(self, reps=3):
         _check(lib().tara_ens_tune(self._h, reps))
@@ -656,6 +684,10 @@ class MhdSolver:
     def ic_abc(self, rho0=1.0, u0=0.1, b0=0.1, p0=1.0, a=1.0, b=1.0, c=1.0, kf=1.0):
         """The reference's ABC-flow initial state (src/main3com.f95:290-324) on the device."""
         _check(lib().tara_mhd_ic_abc(self._h, rho0, u0, b0, p0, a, b, c, kf))
+
+    def ic_random(self, seed=1, k0=3.0, e_kin=1.0, e_mag=0.5):
+        """Random-phase solenoidal u and B on the device (tara_mhd_ic_random)."""
+        _check(lib().tara_mhd_ic_random(self._h, seed, k0, e_kin, e_mag))
 
     def sync(self):
         _check(lib().tara_mhd_sync(self._h))

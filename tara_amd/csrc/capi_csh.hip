@@ -270,6 +270,8 @@ int csh_ctx_zero_history(tara_csh* c)
 {
     TARA_CHECK(c, TARA_ERR_ARG, "null context");
     TARA_HIP(hipMemsetAsync(c->aux.p, 0, c->aux.bytes, c->st));
+    int rc = csh_clear_nonfinite(c->plan, -1, c->st);  // a new state: the guard starts over
+    if (rc) return rc;
     TARA_HIP(hipStreamSynchronize(c->st));
     return TARA_OK;
 }
@@ -282,10 +284,20 @@ extern "C" int tara_csh_upload(tara_csh* c, const double* omegak)
     TARA_HIP(hipMemcpyAsync(c->stage.p, omegak, c->stage.bytes, hipMemcpyHostToDevice, c->st));
     int rc = to_kymajor(c->stage.as<double2>(), c->w.as<double2>(), p.nh, p.ny, 1, c->st);
     if (rc) return rc;
-    TARA_HIP(hipMemsetAsync(c->aux.p, 0, c->aux.bytes, c->st));
-    TARA_HIP(hipStreamSynchronize(c->st));
+    return csh_ctx_zero_history(c);
+}
+
+namespace {
+// The guard of the resident steppers: the last stage of every step flags a
+// non-finite new state value; reported here after the stream has drained.
+int check_finite(tara_csh* c)
+{
+    const int n = csh_nonfinite(c->plan, nullptr, c->st);
+    if (n < 0) return n;
+    TARA_CHECK(n == 0, TARA_ERR_NONFINITE, "the state became non-finite (NaN or Inf) during advance");
     return TARA_OK;
 }
+}  // namespace
 
 extern "C" int tara_csh_download(tara_csh* c, double* omegak)
 {
@@ -295,7 +307,7 @@ extern "C" int tara_csh_download(tara_csh* c, double* omegak)
     if (rc) return rc;
     TARA_HIP(hipMemcpyAsync(omegak, c->stage.p, c->stage.bytes, hipMemcpyDeviceToHost, c->st));
     TARA_HIP(hipStreamSynchronize(c->st));
-    return TARA_OK;
+    return check_finite(c);  // the state is copied out either way
 }
 
 extern "C" int tara_csh_advance(tara_csh* c, int nsteps)
@@ -325,7 +337,7 @@ extern "C" int tara_csh_sync(tara_csh* c)
 {
     TARA_CHECK(c, TARA_ERR_ARG, "null context");
     TARA_HIP(hipStreamSynchronize(c->st));
-    return TARA_OK;
+    return check_finite(c);
 }
 
 extern "C" int tara_csh_destroy(tara_csh* c)

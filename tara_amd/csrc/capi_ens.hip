@@ -226,6 +226,7 @@ extern "C" int tara_ens_upload(tara_ens* e, int run, const double* omegak)
     if ((rc = to_kymajor(b->stage.as<double2>(), b->w.as<double2>() + slot * p.bstride, p.nh, p.ny, 1, nullptr)))
         return rc;
     TARA_HIP(hipMemset(b->aux.as<double2>() + slot * p.bstride, 0, p.bstride * sizeof(double2)));
+    if ((rc = csh_clear_nonfinite(p, slot, nullptr))) return rc;
     TARA_HIP(hipDeviceSynchronize());
     return TARA_OK;
 }
@@ -253,6 +254,7 @@ extern "C" int tara_ens_ic_random(tara_ens* e, int run, unsigned seed, double k0
     if ((rc = csh_random_phase(b->w.as<double2>() + slot * p.bstride, b->n, b->n, seed, k0, energy, e->st)))
         return rc;
     TARA_HIP(hipMemsetAsync(b->aux.as<double2>() + slot * p.bstride, 0, p.bstride * sizeof(double2), e->st));
+    if ((rc = csh_clear_nonfinite(p, slot, e->st))) return rc;
     TARA_HIP(hipStreamSynchronize(e->st));
     return TARA_OK;
 }
@@ -273,7 +275,7 @@ extern "C" int tara_ens_set_omega(tara_ens* e, int run, const float* omega)
                        b->spec.as<double2>(), ld, p.nh, n, b->w.as<double2>() + slot * p.bstride);
     TARA_HIP(hipGetLastError());
     TARA_HIP(hipMemsetAsync(b->aux.as<double2>() + slot * p.bstride, 0, p.bstride * sizeof(double2), e->st));
-    return TARA_OK;
+    return csh_clear_nonfinite(p, slot, e->st);
 }
 
 extern "C" int tara_ens_get_omega(tara_ens* e, int run, float* omega)
@@ -299,8 +301,10 @@ extern "C" int tara_ens_advance(tara_ens* e, int nsteps)
 {
     TARA_CHECK(e && nsteps >= 0, TARA_ERR_ARG, "bad argument");
     int rc;
-    if (nsteps == 0) return TARA_OK;
+    // the step graph is captured on the first call, also with nsteps = 0 (a
+    // caller can then capture before any work is in flight)
     if (e->use_graph && !e->exec && (rc = capture(e))) return rc;
+    if (nsteps == 0) return TARA_OK;
     if ((rc = enqueue_head(e))) return rc;
     for (int s = 0; s < nsteps; ++s) {
         if (e->use_graph) {
@@ -312,10 +316,53 @@ extern "C" int tara_ens_advance(tara_ens* e, int nsteps)
     return TARA_OK;
 }
 
+extern "C" int tara_ens_graph_nodes(tara_ens* e, long* nodes, long* kernels)
+{
+    TARA_CHECK(e && nodes, TARA_ERR_ARG, "null argument");
+    *nodes = 0;
+    if (kernels) *kernels = 0;
+    if (!e->graph) return TARA_OK;
+    size_t n = 0;
+    TARA_HIP(hipGraphGetNodes(e->graph, nullptr, &n));
+    *nodes = (long)n;
+    if (kernels && n) {
+        std::vector<hipGraphNode_t> v(n);
+        TARA_HIP(hipGraphGetNodes(e->graph, v.data(), &n));
+        long k = 0;
+        for (size_t i = 0; i < n; ++i) {
+            hipGraphNodeType t;
+            TARA_HIP(hipGraphNodeGetType(v[i], &t));
+            k += t == hipGraphNodeTypeKernel ? 1 : 0;
+        }
+        *kernels = k;
+    }
+    return TARA_OK;
+}
+
+extern "C" int tara_ens_nonfinite(tara_ens* e, int* flags)
+{
+    TARA_CHECK(e, TARA_ERR_ARG, "null context");
+    int total = 0;
+    std::vector<int> f;
+    for (auto& bp : e->b) {
+        Bucket& b = *bp;
+        f.assign(b.runs.size(), 0);
+        const int n = csh_nonfinite(b.plan, f.data(), e->st);
+        if (n < 0) return n;
+        total += n;
+        if (flags)
+            for (size_t k = 0; k < b.runs.size(); ++k) flags[b.runs[k]] = f[k];
+    }
+    return total;
+}
+
 extern "C" int tara_ens_sync(tara_ens* e)
 {
     TARA_CHECK(e, TARA_ERR_ARG, "null context");
     TARA_HIP(hipStreamSynchronize(e->st));
+    const int n = tara_ens_nonfinite(e, nullptr);
+    if (n < 0) return n;
+    TARA_CHECK(n == 0, TARA_ERR_NONFINITE, "%d run(s) of the ensemble became non-finite (NaN or Inf)", n);
     return TARA_OK;
 }
 

@@ -9,6 +9,7 @@
 
 #include "fft_api.h"
 #include "mhd3d.h"
+#include "philox.h"
 
 namespace tara {
 namespace {
@@ -160,9 +161,23 @@ extern "C" int tara_mhd_upload(tara_mhd* c, const double* const state[8])
         if ((rc = h2d(c->plan, c->st[f].as<double2>(), state[f], c->s))) return rc;
     }
     for (int f = 0; f < 7; ++f) TARA_HIP(hipMemsetAsync(c->hist[f].p, 0, c->hist[f].bytes, c->s));
+    TARA_HIP(hipMemsetAsync(c->plan.bad.p, 0, sizeof(int), c->s));
     TARA_HIP(hipStreamSynchronize(c->s));
     return TARA_OK;
 }
+
+namespace {
+// The guard of the resident stepper: the AB2 update flags a non-finite new
+// state value; reported after the stream has drained.
+int check_finite(tara_mhd* c)
+{
+    int bad = 0;
+    TARA_HIP(hipMemcpyAsync(&bad, c->plan.bad.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    TARA_HIP(hipStreamSynchronize(c->s));
+    TARA_CHECK(bad == 0, TARA_ERR_NONFINITE, "the 3D state became non-finite (NaN or Inf) during advance");
+    return TARA_OK;
+}
+}  // namespace
 
 extern "C" int tara_mhd_download(tara_mhd* c, double* const state[8])
 {
@@ -173,7 +188,7 @@ extern "C" int tara_mhd_download(tara_mhd* c, double* const state[8])
         if ((rc = d2h(c->plan, state[f], c->st[f].as<double2>(), c->s))) return rc;
     }
     TARA_HIP(hipStreamSynchronize(c->s));
-    return TARA_OK;
+    return check_finite(c);  // the state is copied out either way
 }
 
 extern "C" int tara_mhd_advance(tara_mhd* c, int nsteps)
@@ -195,7 +210,7 @@ extern "C" int tara_mhd_sync(tara_mhd* c)
 {
     TARA_CHECK(c, TARA_ERR_ARG, "null context");
     TARA_HIP(hipStreamSynchronize(c->s));
-    return TARA_OK;
+    return check_finite(c);
 }
 
 namespace {
@@ -251,7 +266,193 @@ extern "C" int tara_mhd_ic_abc(tara_mhd* c, double rho0, double u0, double b0, d
         if ((rc = col_fft(s, p.nz, p.ld, (long)p.ld * p.ny, p.ld, p.ny, -1, c->s))) return rc;
     }
     for (int f = 0; f < 7; ++f) TARA_HIP(hipMemsetAsync(c->hist[f].p, 0, c->hist[f].bytes, c->s));
+    TARA_HIP(hipMemsetAsync(c->plan.bad.p, 0, sizeof(int), c->s));
     TARA_HIP(hipStreamSynchronize(c->s));
+    return TARA_OK;
+}
+
+namespace {
+
+constexpr int kIcBlocks = 1024;  // grid of the generation kernel (fixed: its partial sums are reduced in order)
+
+// One random-phase solenoidal vector field (v0, v1, v2: padded (ld, Ny, Nz)
+// half spectra) with |v(k)| per component k exp(-(k/k0)^2) before the
+// projection v <- v - k (k . v)/k^2, i.e. shell spectrum ~ k^4 exp(-2 (k/k0)^2).
+// Mode (i, j, l) draws its three phases 2 pi u from Philox-4x32-10 with key
+// {seed, tag} and counters {m, 0, 0, 0} (u_0: words 0-1, u_1: words 2-3) and
+// {m, 1, 0, 0} (u_2: words 0-1), m = i + Nh (j + Ny l); on the kx = 0 plane a
+// mode whose mirror (-ky, -kz) has the smaller index j + Ny l takes the
+// conjugate of the mirror's draw.  k = 0, the Nyquist planes and the padding
+// columns are zero.  Each block writes its share of sum w |v|^2 (w = 1 on
+// kx = 0, 2 elsewhere) to partial[blockIdx.x].
+__global__ __launch_bounds__(256) void k_ic3d_field(double2* v0, double2* v1, double2* v2, int nh, int ny, int nz,
+                                                   int ld, unsigned seed, unsigned tag, double k0, double* partial)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    const size_t n = (size_t)ld * ny * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % ld), j = (int)((q / ld) % ny), l = (int)(q / ((size_t)ld * ny));
+        double2 o[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+        if (!(i >= nh - 1 || j == ny / 2 || l == nz / 2 || (i == 0 && j == 0 && l == 0))) {
+            int jc = j, lc = l;
+            bool conj = false;
+            if (i == 0) {
+                const int jm = (ny - j) % ny, lm = (nz - l) % nz;
+                if (jm + ny * lm < j + ny * l) {
+                    jc = jm;
+                    lc = lm;
+                    conj = true;
+                }
+            }
+            const double kv[3] = {(double)i, (double)(j < ny / 2 ? j : j - ny), (double)(l < nz / 2 ? l : l - nz)};
+            const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+            const double k = sqrt(k2);
+            const double amp = k * exp(-(k / k0) * (k / k0));
+            const unsigned m = (unsigned)((size_t)i + (size_t)nh * ((size_t)jc + (size_t)ny * lc));
+            const U4 r = philox4x32_10(U4{m, 0u, 0u, 0u}, seed, tag);
+            const U4 t = philox4x32_10(U4{m, 1u, 0u, 0u}, seed, tag);
+            const double u[3] = {u53(r.x, r.y), u53(r.z, r.w), u53(t.x, t.y)};
+            double re[3], im[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double ph = 2.0 * 3.14159265358979323846 * u[c];
+                re[c] = amp * cos(ph);
+                im[c] = conj ? -(amp * sin(ph)) : amp * sin(ph);
+            }
+            const double dr = (kv[0] * re[0] + kv[1] * re[1] + kv[2] * re[2]) / k2;
+            const double di = (kv[0] * im[0] + kv[1] * im[1] + kv[2] * im[2]) / k2;
+            const double w = i == 0 ? 1.0 : 2.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                o[c] = make_double2(re[c] - kv[c] * dr, im[c] - kv[c] * di);
+                s += w * (o[c].x * o[c].x + o[c].y * o[c].y);
+            }
+        }
+        v0[q] = o[0];
+        v1[q] = o[1];
+        v2[q] = o[2];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// scale[0] = sqrt(energy / e) with e = 1/2 sum(partial) / nn^2, the field's
+// 1/2 <|v|^2> (Parseval over the full spectrum of an unnormalised transform).
+__global__ __launch_bounds__(256) void k_ic3d_scale_factor(const double* partial, int np, double energy, double nn,
+                                                          double* scale)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < np; b += blockDim.x) s += partial[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double e = 0.5 * red[0] / (nn * nn);
+        scale[0] = e > 0 ? sqrt(energy / e) : 0.0;
+    }
+}
+
+__global__ void k_ic3d_rescale(double2* v0, double2* v1, double2* v2, size_t n, const double* scale)
+{
+    const double f = scale[0];
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const double2 a = v0[q], b = v1[q], c = v2[q];
+        v0[q] = make_double2(a.x * f, a.y * f);
+        v1[q] = make_double2(b.x * f, b.y * f);
+        v2[q] = make_double2(c.x * f, c.y * f);
+    }
+}
+
+// acc = (first ? 0 : acc) + x^2
+__global__ void k_ic3d_accum(double* acc, const double* x, size_t n, int first)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const double v = x[q];
+        acc[q] = (first ? 0.0 : acc[q]) + v * v;
+    }
+}
+
+// E = P + rho u^2/2 + B^2/2 with P = 1, rho = 1: acc -> 1 + acc/2 in place
+__global__ void k_ic3d_energy(double* acc, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        acc[q] = 1.0 + 0.5 * acc[q];
+}
+
+// rho = 1: the unnormalised transform is Nx Ny Nz at k = 0 and zero elsewhere.
+__global__ void k_ic3d_density(double2* r, size_t n, double nn)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        r[q] = make_double2(q == 0 ? nn : 0.0, 0.0);
+}
+
+unsigned grid_of(size_t n)
+{
+    const size_t b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b ? b : 1));
+}
+
+}  // namespace
+
+extern "C" int tara_mhd_ic_random(tara_mhd* c, unsigned seed, double k0, double e_kin, double e_mag)
+{
+    TARA_CHECK(c, TARA_ERR_ARG, "null context");
+    TARA_CHECK(k0 > 0 && e_kin >= 0 && e_mag >= 0, TARA_ERR_ARG, "ic: k0 must be positive, energies non-negative");
+    const MhdPlan& p = c->plan;
+    const size_t n = p.fsz, nr = (size_t)p.nx * p.ny * p.nz;
+    const double nn = (double)nr;
+    hipStream_t s = c->s;
+    // the history buffers are zeroed at the end and serve as scratch meanwhile:
+    // hist[0] partial sums + scale, hist[1] spectral copy, hist[2] real field,
+    // hist[3] the accumulated u^2 + B^2
+    double* partial = c->hist[0].as<double>();
+    double2* spec = c->hist[1].as<double2>();
+    double* real = c->hist[2].as<double>();
+    double* acc = c->hist[3].as<double>();
+    const unsigned tags[2] = {1u, 2u};
+    const double energies[2] = {e_kin, e_mag};
+    const int base[2] = {1, 5};
+    for (int v = 0; v < 2; ++v) {
+        double2* f0 = c->st[base[v]].as<double2>();
+        double2* f1 = c->st[base[v] + 1].as<double2>();
+        double2* f2 = c->st[base[v] + 2].as<double2>();
+        hipLaunchKernelGGL(k_ic3d_field, dim3(kIcBlocks), dim3(256), 0, s, f0, f1, f2, p.nh, p.ny, p.nz, p.ld, seed,
+                           tags[v], k0, partial);
+        hipLaunchKernelGGL(k_ic3d_scale_factor, dim3(1), dim3(256), 0, s, partial, kIcBlocks, energies[v], nn,
+                           partial + kIcBlocks);
+        hipLaunchKernelGGL(k_ic3d_rescale, dim3(grid_of(n)), dim3(256), 0, s, f0, f1, f2, n, partial + kIcBlocks);
+        TARA_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_ic3d_density, dim3(grid_of(n)), dim3(256), 0, s, c->st[0].as<double2>(), n, nn);
+    int rc, first = 1;
+    for (int f : {1, 2, 3, 5, 6, 7}) {
+        TARA_HIP(hipMemcpyAsync(spec, c->st[f].p, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        if ((rc = col_fft(spec, p.nz, p.ld, (long)p.ld * p.ny, p.ld, p.ny, +1, s))) return rc;
+        if ((rc = col_fft(spec, p.ny, p.ld, p.ld, (long)p.ld * p.ny, p.nz, +1, s))) return rc;
+        if ((rc = row_c2r(spec, real, p.nx, p.ld, p.ny * p.nz, 1.0 / nn, s))) return rc;
+        hipLaunchKernelGGL(k_ic3d_accum, dim3(grid_of(nr)), dim3(256), 0, s, acc, real, nr, first);
+        first = 0;
+    }
+    hipLaunchKernelGGL(k_ic3d_energy, dim3(grid_of(nr)), dim3(256), 0, s, acc, nr);
+    TARA_HIP(hipGetLastError());
+    double2* e = c->st[4].as<double2>();
+    TARA_HIP(hipMemsetAsync(e, 0, c->st[4].bytes, s));
+    if ((rc = row_r2c(acc, e, p.nx, p.ld, p.ny * p.nz, s))) return rc;
+    if ((rc = col_fft(e, p.ny, p.ld, p.ld, (long)p.ld * p.ny, p.nz, -1, s))) return rc;
+    if ((rc = col_fft(e, p.nz, p.ld, (long)p.ld * p.ny, p.ld, p.ny, -1, s))) return rc;
+    for (int f = 0; f < 7; ++f) TARA_HIP(hipMemsetAsync(c->hist[f].p, 0, c->hist[f].bytes, s));
+    TARA_HIP(hipMemsetAsync(c->plan.bad.p, 0, sizeof(int), s));
+    TARA_HIP(hipStreamSynchronize(s));
     return TARA_OK;
 }
 

@@ -67,7 +67,13 @@ struct CshPlan {
     const double2* tw_xn16 = nullptr;
     int xcfg = 0;                    // x-pass shape (see csh_xshape_ok / the autotuner)
     const double2* rctw = nullptr;
+    DevBuf bad;                      // int per run: set when a step's new state holds a non-finite value
 };
+// The per-run non-finite flags of a plan: csh_nonfinite synchronises st and
+// returns the number of flagged runs (flags[r] = 0 / 1 when flags is not null);
+// csh_clear_nonfinite clears run r's flag (every run's for r < 0), ordered on st.
+int csh_nonfinite(const CshPlan& p, int* flags, hipStream_t st);
+int csh_clear_nonfinite(const CshPlan& p, int r, hipStream_t st);
 
 // Physics of one right-hand side.  screened = 0: derive2csh (psi = omega/k^2,
 // derivatives and viscous term on omega).  screened = 1: derive2css

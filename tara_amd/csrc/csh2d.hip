@@ -310,6 +310,7 @@ struct YfwdArgs {
     int screened;
     size_t bs;             // batched runs: field stride per run (blockIdx.y)
     const double* nus;     // batched runs: viscosity per run (null: nu)
+    int* bad;              // per run: set to 1 when a new state value is not finite (AB2, RK4 stage 4)
 };
 
 template <int NY, int MODE>
@@ -335,6 +336,7 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const d
     rf_fft<NY, -1>(v, t, lds + c * RF<NY>::LDS, tw);
     if (!live) return;
     const double kxv = (double)i;
+    bool finite = true;  // the new state's values (AB2, RK4 stage 4)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int j = rf_final_idx<NY>(t, e);
@@ -359,8 +361,10 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const d
             a.next[q] = f;
         } else if (MODE == CSH_AB2) {
             const double2 fo = a.aux[q];
-            a.base[q] = make_double2(ws.x + (1.5 * f.x - 0.5 * fo.x) * a.dt, ws.y + (1.5 * f.y - 0.5 * fo.y) * a.dt);
+            const double2 wn = make_double2(ws.x + (1.5 * f.x - 0.5 * fo.x) * a.dt, ws.y + (1.5 * f.y - 0.5 * fo.y) * a.dt);
+            a.base[q] = wn;
             a.aux[q] = f;
+            finite = finite && isfinite(wn.x) && isfinite(wn.y);
         } else if (MODE == CSH_RK4_1) {
             const double c6 = a.dt / 6.0, c2 = 0.5 * a.dt;
             a.aux[q] = make_double2(ws.x + c6 * f.x, ws.y + c6 * f.y);
@@ -373,9 +377,12 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const d
         } else {  // CSH_RK4_4
             const double c6 = a.dt / 6.0;
             const double2 acc = a.aux[q];
-            a.base[q] = make_double2(acc.x + c6 * f.x, acc.y + c6 * f.y);
+            const double2 wn = make_double2(acc.x + c6 * f.x, acc.y + c6 * f.y);
+            a.base[q] = wn;
+            finite = finite && isfinite(wn.x) && isfinite(wn.y);
         }
     }
+    if (!finite && a.bad) a.bad[blockIdx.y] = 1;
 }
 
 // ------------------------------------------------------- passes 3 + 1 fused
@@ -422,6 +429,7 @@ __global__ __launch_bounds__((YiShape<NY, false>::NT), (YiShape<NY, false>::IL ?
     for (int e = 0; e < E; ++e) v[e] = live ? a.h[blk8(i, rf_first_idx<NY>(t, e), a.nh)] : make_double2(0.0, 0.0);
     rf_fft<NY, -1>(v, t, xl, tw);
     const double kxv = (double)i;
+    bool finite = true;  // the new state's values (AB2, RK4 stage 4)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int j = rf_final_idx<NY>(t, e);
@@ -440,6 +448,7 @@ __global__ __launch_bounds__((YiShape<NY, false>::NT), (YiShape<NY, false>::IL ?
                 a.base[q] = wn;
                 a.aux[q] = f;
             }
+            finite = finite && isfinite(wn.x) && isfinite(wn.y);
         } else if (MODE == CSH_RK4_1) {
             const double c6 = a.dt / 6.0, c2 = 0.5 * a.dt;
             wn = make_double2(ws.x + c2 * f.x, ws.y + c2 * f.y);
@@ -461,9 +470,11 @@ __global__ __launch_bounds__((YiShape<NY, false>::NT), (YiShape<NY, false>::IL ?
             const double2 acc = live ? a.aux[q] : make_double2(0.0, 0.0);
             wn = make_double2(acc.x + c6 * f.x, acc.y + c6 * f.y);
             if (live) a.base[q] = wn;
+            finite = finite && isfinite(wn.x) && isfinite(wn.y);
         }
         v[e] = wn;
     }
+    if (!finite && a.bad) a.bad[blockIdx.y] = 1;
     if constexpr (!rf_final_is_first<NY, 8>()) rf_final_to_first<NY>(v, t, xl);
     csh_yinv_column<NY, false, GS, S::IL>(v, i, live, t, xl, tw, 0.0, ld, g0, g1, g2, g3);
 }
@@ -657,6 +668,8 @@ int csh_plan_init(CshPlan& p, int nx, int ny, int batch)
         TARA_HIP(hipMemset(p.g[f].p, 0, batch * ns * sizeof(double2)));
     }
     if ((rc = p.h.alloc(batch * ns * sizeof(double2)))) return rc;
+    if ((rc = p.bad.alloc(batch * sizeof(int)))) return rc;
+    TARA_HIP(hipMemset(p.bad.p, 0, batch * sizeof(int)));
     // wavenumber and dealias tables, with derive2csh's own expressions
     std::vector<double> kx(p.nh), ky(ny);
     std::vector<int> imin(ny);
@@ -709,6 +722,30 @@ int csh_xshape_ok(const CshPlan& p, int c) { return c >= 0 && c < kNumXCands && 
 
 int csh_xpass(const CshPlan& p, hipStream_t st) { return xpass(p, st); }
 
+int csh_nonfinite(const CshPlan& p, int* flags, hipStream_t st)
+{
+    std::vector<int> f(p.batch, 0);
+    TARA_HIP(hipMemcpyAsync(f.data(), p.bad.p, p.batch * sizeof(int), hipMemcpyDeviceToHost, st));
+    TARA_HIP(hipStreamSynchronize(st));
+    int n = 0;
+    for (int r = 0; r < p.batch; ++r) {
+        n += f[r] != 0;
+        if (flags) flags[r] = f[r] != 0;
+    }
+    return n;
+}
+
+int csh_clear_nonfinite(const CshPlan& p, int r, hipStream_t st)
+{
+    if (r < 0) {
+        TARA_HIP(hipMemsetAsync(p.bad.p, 0, p.batch * sizeof(int), st));
+    } else {
+        TARA_CHECK(r < p.batch, TARA_ERR_ARG, "run %d out of range", r);
+        TARA_HIP(hipMemsetAsync(p.bad.as<int>() + r, 0, sizeof(int), st));
+    }
+    return TARA_OK;
+}
+
 int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& ph, double dt, double2* base,
                   double2* aux, double2* next, hipStream_t st, EventTimer* tm)
 {
@@ -736,6 +773,7 @@ int csh_rhs_stage(const CshPlan& p, const double2* ws, int mode, const CshPhys& 
     a.screened = ph.screened;
     a.bs = p.bstride;
     a.nus = ph.nus;
+    a.bad = p.bad.as<int>();
     if (tm) tm->begin(2, st);
     if ((rc = yfwd(p, mode, a, st))) return rc;
     if (tm) tm->end(2, st);
@@ -784,6 +822,7 @@ int csh_fused_step(const CshPlan& p, int scheme, const CshPhys& ph, double dt, d
     a.screened = 0;
     a.bs = p.bstride;
     a.nus = ph.nus;
+    a.bad = p.bad.as<int>();
     const int nst = scheme == TARA_SCHEME_AB2 ? 1 : 4;
     for (int k = 0; k < nst; ++k) {
         const int mode = scheme == TARA_SCHEME_AB2 ? CSH_AB2 : CSH_RK4_1 + k;

@@ -306,6 +306,7 @@ struct tara_fno {
     hipGraphExec_t gexec = nullptr;
     int gbatch = 0;
     std::vector<hipEvent_t> ev_in, ev_done;
+    hipEvent_t ev_anchor = nullptr;  // the anchor solver's take-over is done
 };
 
 namespace {
@@ -921,8 +922,8 @@ int rollout_step(tara_fno* f, int nb, const float* in, float* out, bool graph)
 }  // namespace
 
 extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps, int every, int nens,
-                                tara_ens* const* checkers, int solver_steps, float tol, int use_graph,
-                                float* check_err, int* fallbacks)
+                                tara_ens* const* checkers, tara_ens* anchor, int solver_steps, float tol,
+                                int use_graph, float* check_err, int* fallbacks)
 {
     const int nchk = every >= 1 ? nsteps / every : 0;
     // one ensemble per check (started as soon as its input exists), or one
@@ -936,6 +937,8 @@ extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps,
         TARA_CHECK(checkers[j] && ens_grid_of(checkers[j], runs - 1) == f->n, TARA_ERR_ARG,
                    "fno rollout: checker %d needs %d runs of grid %d", j, runs, f->n);
     }
+    TARA_CHECK(nchk == 0 || (anchor && ens_grid_of(anchor, batch - 1) == f->n), TARA_ERR_ARG,
+               "fno rollout: the anchor solver needs %d runs of grid %d", batch, f->n);
     const size_t np = f->np, per = (size_t)batch * np;
     int rc;
     if (nchk > 0) {
@@ -951,10 +954,19 @@ extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps,
             f->ev_in.push_back(a);
             f->ev_done.push_back(b);
         }
-        // capture the checkers' step graphs before any of them is in flight
+        if (!f->ev_anchor) TARA_HIP(hipEventCreateWithFlags(&f->ev_anchor, hipEventDisableTiming));
+        // capture the checkers' and the anchor's step graphs before any of them is in flight
         for (int j = 0; j < nens; ++j)
             if ((rc = tara_ens_advance(checkers[j], 0))) return rc;
+        if ((rc = tara_ens_advance(anchor, 0))) return rc;
+        // the anchor holds the last state known to be the FP64 solution (traj[0]
+        // here) in FP64, at step ta of the rollout
+        TARA_HIP(hipEventRecord(f->ev_anchor, f->st));
+        TARA_HIP(hipStreamWaitEvent(ens_stream(anchor), f->ev_anchor, 0));
+        for (int b = 0; b < batch; ++b)
+            if ((rc = tara_ens_set_omega(anchor, b, traj + b * np))) return rc;
     }
+    int ta = 0;
     const bool graph = use_graph != 0;
     if (graph && f->gbatch != batch && (rc = capture_step(f, batch))) return rc;
     std::vector<float> err((size_t)nchk * batch, 0.f), fresh(err.size());
@@ -1009,7 +1021,12 @@ extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps,
         TARA_HIP(hipMemcpyAsync(fresh.data(), f->cerr.p, fresh.size() * 4, hipMemcpyDeviceToHost, f->st));
         TARA_HIP(hipStreamSynchronize(f->st));
         for (size_t q = (size_t)first_chk * batch; q < err.size(); ++q) err[q] = fresh[q];
-        // the earliest failing check hands its step to the solver; the rest is redone from there
+        // At the earliest failing check the FP64 solver takes the trajectory
+        // over: the anchor advances from its step ta to the failing step tf and
+        // its fields replace traj[tf] for every sample of the batch, so every
+        // state after a take-over is the FP64 solution from traj[0], not a
+        // correction of an already drifted surrogate state.  The rollout is
+        // then redone from tf, and tf becomes the anchor's step.
         int fail = -1;
         for (int j = first_chk; j < nchk && fail < 0; ++j)
             for (int b = 0; b < batch; ++b)
@@ -1017,11 +1034,12 @@ extern "C" int tara_fno_rollout(tara_fno* f, int batch, float* traj, int nsteps,
         if (fail < 0) break;
         ++nfall;
         const int tf = (fail + 1) * every;
+        if ((rc = tara_ens_advance(anchor, (tf - ta) * solver_steps))) return rc;
         for (int b = 0; b < batch; ++b)
-            if (!(err[(size_t)fail * batch + b] <= tol))
-                TARA_HIP(hipMemcpyAsync(traj + (size_t)tf * per + b * np,
-                                        f->chk.as<float>() + (size_t)fail * per + b * np, np * 4,
-                                        hipMemcpyDeviceToDevice, f->st));
+            if ((rc = tara_ens_get_omega(anchor, b, traj + (size_t)tf * per + b * np))) return rc;
+        TARA_HIP(hipEventRecord(f->ev_anchor, ens_stream(anchor)));
+        TARA_HIP(hipStreamWaitEvent(f->st, f->ev_anchor, 0));
+        ta = tf;
         s = tf;
     }
     TARA_HIP(hipStreamSynchronize(f->st));
@@ -1040,6 +1058,7 @@ extern "C" int tara_fno_destroy(tara_fno* f)
     drop_fno_graph(f);
     for (auto e : f->ev_in) (void)hipEventDestroy(e);
     for (auto e : f->ev_done) (void)hipEventDestroy(e);
+    if (f->ev_anchor) (void)hipEventDestroy(f->ev_anchor);
     delete f;
     return TARA_OK;
 }

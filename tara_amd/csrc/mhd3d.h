@@ -19,6 +19,7 @@ struct MhdPlan {
     DevBuf a;                  // 7 fields: inverse-transformed state (rho, rho u, B)
     DevBuf b;                  // 14 fields: u (3), momentum fluxes (8), induction fluxes (3)
     DevBuf kx, ky, kz, imin;
+    DevBuf bad;                // int: set when an AB2 step's new state holds a non-finite value
     const double2* tw_y = nullptr;
     const double2* tw_z = nullptr;
     const double2* tw_z4 = nullptr;  // radix cap 4 (z forward pass at 4 elements per thread)

@@ -154,6 +154,7 @@ struct ZArgs {
     const int* imin;   // per (k, j) row: modes with i >= imin are dealiased away
     int ny, ld;
     double mu, eta, dt;
+    int* bad;          // AB2: set to 1 when a new state value is not finite
 };
 
 // The z pass holds 14 transforms' worth of accumulators; at 4 elements per
@@ -206,6 +207,7 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
     // anyway, so the momentum spectra are read once (not again for the density
     // term before the 14 transforms); density is finished after field 3.
     double2 dens[E];
+    bool finite = true;  // AB2: every new state value of this lane is finite
     // wpre: the state of field fld at this lane's positions when the caller
     // has already loaded it (AB2), else null.
     auto finish = [&](int fld, const double2 (&acc)[E], const double2* wpre = nullptr) {
@@ -219,8 +221,10 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
                 if (fld == 1) dens[e] = ik(kxv, w);
                 if (fld == 2) { const double2 s2 = ik(kyv, w); dens[e] = make_double2(dens[e].x + s2.x, dens[e].y + s2.y); }
                 if (fld == 3) { const double2 s3 = ik(kzv[e], w); dens[e] = make_double2(dens[e].x + s3.x, dens[e].y + s3.y); }
-                a.st[fld][q[e]] = make_double2(w.x + (1.5 * f.x - 0.5 * fo.x) * a.dt, w.y + (1.5 * f.y - 0.5 * fo.y) * a.dt);
+                const double2 wn = make_double2(w.x + (1.5 * f.x - 0.5 * fo.x) * a.dt, w.y + (1.5 * f.y - 0.5 * fo.y) * a.dt);
+                a.st[fld][q[e]] = wn;
                 a.out[fld][q[e]] = f;
+                finite = finite && isfinite(wn.x) && isfinite(wn.y);
             }
         }
     };
@@ -314,6 +318,7 @@ __global__ __launch_bounds__((CShape<NZ, EM>::NT), 4) void k_mhd_zfwd(ZArgs a, c
     }
     finish(5, acc2, bs);
     finish(6, acc3, bz);
+    if (MODE == MHD_AB2 && !finite && a.bad) a.bad[0] = 1;
 }
 
 // ------------------------------------------------------------------ dispatch
@@ -419,8 +424,10 @@ int mhd_plan_init(MhdPlan& p, int nx, int ny, int nz, double lx, double ly, doub
             imin[(size_t)k * ny + j] = m;  // kx increases with i: every i >= m is cut
         }
     if ((rc = p.kx.alloc(p.ld * sizeof(double))) || (rc = p.ky.alloc(ny * sizeof(double))) ||
-        (rc = p.kz.alloc(nz * sizeof(double))) || (rc = p.imin.alloc(imin.size() * sizeof(int))))
+        (rc = p.kz.alloc(nz * sizeof(double))) || (rc = p.imin.alloc(imin.size() * sizeof(int))) ||
+        (rc = p.bad.alloc(sizeof(int))))
         return rc;
+    TARA_HIP(hipMemset(p.bad.p, 0, sizeof(int)));
     TARA_HIP(hipMemcpy(p.kx.p, kx.data(), p.ld * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.ky.p, ky.data(), ny * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.kz.p, kz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
@@ -484,6 +491,7 @@ int mhd_rhs_stage(const MhdPlan& p, double2* const st[7], int mode, double dt, d
     z.mu = p.mu;
     z.eta = p.eta;
     z.dt = dt;
+    z.bad = p.bad.as<int>();
     if (tm) tm->begin(3, s);
     if ((rc = zpass(p, mode, z, s))) return rc;
     if (tm) tm->end(3, s);

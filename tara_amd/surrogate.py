@@ -108,6 +108,7 @@ def rollout(net, ref, nu, dt, steps, every=10, tol=0.05, graph=True, reps=1, bat
         checkers = [Ensemble([n] * nchk, [nu] * nchk, dt, SCHEME_RK4)]
     else:
         checkers = [Ensemble([n], [nu], dt, SCHEME_RK4) for _ in range(nchk)]
+    anchor = Ensemble([n], [nu], dt, SCHEME_RK4) if nchk > 0 else None
     traj = torch.empty((s1, 1, n, n), dtype=torch.float32, device="cuda")
     out = {"final": [], "max": [], "check_err": [], "fallbacks": [], "ms": []}
     try:
@@ -117,7 +118,7 @@ def rollout(net, ref, nu, dt, steps, every=10, tol=0.05, graph=True, reps=1, bat
                 traj[0, 0].copy_(ref[0, r])
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                err, nf = net.rollout(1, traj, nsteps, every, checkers, steps, tol, graph)
+                err, nf = net.rollout(1, traj, nsteps, every, checkers, steps, tol, graph, anchor=anchor)
                 torch.cuda.synchronize()
                 ms = (time.perf_counter() - t0) * 1e3
                 best = ms if best is None else min(best, ms)
@@ -129,7 +130,7 @@ def rollout(net, ref, nu, dt, steps, every=10, tol=0.05, graph=True, reps=1, bat
             out["fallbacks"].append(nf)
             out["ms"].append(best)
     finally:
-        for c in checkers:
+        for c in checkers + ([anchor] if anchor is not None else []):
             c.close()
     return out
 

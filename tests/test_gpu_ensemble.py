@@ -38,6 +38,27 @@ def test_ensemble_matches_single_runs():
         assert rel_l2(got[r], want) < 1e-14, r
 
 
+def test_ensemble_with_512_bucket_matches_single_runs():
+    """The benchmark ensemble's largest grid (512^2) batched with the others."""
+    print("ensemble with a 512^2 bucket vs single runs", flush=True)
+    n = [512, 128, 512, 256]
+    nu = [1e-4, 1e-2, 1e-3, 3e-4]
+    w0 = [random_phase_vorticity(m, seed=40 + r, k0=3.0 + 1.5 * r) for r, m in enumerate(n)]
+    e = T.Ensemble(n, nu, 5e-4, T.SCHEME_RK4)
+    for r, w in enumerate(w0):
+        e.upload(r, w)
+    e.advance(10)
+    got = [e.download(r) for r in range(len(n))]
+    e.close()
+    for r, m in enumerate(n):
+        s = T.CshSolver(m, m, nu[r], 5e-4, T.SCHEME_RK4)
+        s.upload(w0[r])
+        s.advance(10)
+        want = s.download()
+        s.close()
+        assert rel_l2(got[r], want) < 1e-14, r
+
+
 def test_graph_replays_match_eager():
     print("ensemble graph vs eager", flush=True)
     n, nu, w0 = _ensemble_inputs()

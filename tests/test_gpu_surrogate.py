@@ -6,9 +6,10 @@ device, and rollouts re-checked by the FP64 solver.
   network, by at least a fifth.
 * The checker's error for a step equals the relative L2 distance between the
   surrogate's step and an independent FP64 solve from the same state.
-* With a tolerance no step meets, every check hands its step to the solver:
-  the checked states are the solver's results from the preceding surrogate
-  states."""
+* With a tolerance no step meets, every check hands the trajectory to the
+  FP64 solver: each checked state is the plain solver's state at that step,
+  advanced from the initial field (the anchor), not a correction of a drifted
+  surrogate state."""
 import numpy as np
 import pytest
 
@@ -66,9 +67,10 @@ def test_checker_errors_match_independent_solves(data, trained):
     net, _ = trained
     nsteps, every = 20, 5
     checkers = [T.Ensemble([N], [NU], DT, T.SCHEME_RK4) for _ in range(nsteps // every)]
+    anchor = T.Ensemble([N], [NU], DT, T.SCHEME_RK4)
     traj = torch.empty((nsteps + 1, 1, N, N), device="cuda")
     traj[0, 0].copy_(held[0, 0])
-    err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=1e9)
+    err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=1e9, anchor=anchor)
     assert nf == 0
     # without take-overs the rollout is the plain network iteration
     plain = torch.empty_like(traj)
@@ -80,26 +82,57 @@ def test_checker_errors_match_independent_solves(data, trained):
         ref = _solve(traj[t - 1, 0], STEPS)
         want = float(torch.linalg.norm(traj[t, 0] - ref) / torch.linalg.norm(ref))
         assert abs(err[j, 0] - want) < 1e-4 * max(want, 1e-3), (j, err[j, 0], want)
-    for c in checkers:
+    for c in checkers + [anchor]:
         c.close()
 
 
-def test_failed_checks_hand_steps_to_the_solver(data, trained):
+@pytest.mark.parametrize("batched", [False, True])
+def test_failed_checks_hand_the_trajectory_to_the_solver(data, trained, batched):
+    """tol = 0: every check fails, and every checked step of the rollout is
+    the plain FP64 solver's state at that step from traj[0]: the take-overs
+    follow one FP64 trajectory instead of correcting drifted surrogate states."""
     _, held = data
     net, _ = trained
     nsteps, every = 20, 5
-    checkers = [T.Ensemble([N], [NU], DT, T.SCHEME_RK4) for _ in range(nsteps // every)]
+    nchk = nsteps // every
+    if batched:
+        checkers = [T.Ensemble([N] * nchk, [NU] * nchk, DT, T.SCHEME_RK4)]
+    else:
+        checkers = [T.Ensemble([N], [NU], DT, T.SCHEME_RK4) for _ in range(nchk)]
+    anchor = T.Ensemble([N], [NU], DT, T.SCHEME_RK4)
     traj = torch.empty((nsteps + 1, 1, N, N), device="cuda")
     traj[0, 0].copy_(held[0, 0])
-    err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=0.0)
-    assert nf == nsteps // every
+    err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=0.0, anchor=anchor)
+    assert nf == nchk
     assert np.all(err > 0)
-    for j in range(nsteps // every):
+    plain = T.Ensemble([N], [NU], DT, T.SCHEME_RK4)
+    plain.set_omega(0, traj[0, 0])
+    out = torch.empty_like(traj[0, 0])
+    for j in range(nchk):
+        plain.advance(every * STEPS)
+        plain.get_omega(0, out)
+        plain.sync()
         t = (j + 1) * every
-        ref = _solve(traj[t - 1, 0], STEPS)
-        assert float(torch.linalg.norm(traj[t, 0] - ref) / torch.linalg.norm(ref)) < 1e-6, j
-    for c in checkers:
+        assert torch.equal(traj[t, 0], out), j
+        # and the surrogate runs on from there
+        assert not torch.equal(traj[t - 1, 0], traj[t, 0])
+    for c in checkers + [anchor, plain]:
         c.close()
+
+
+def test_ensemble_graph_captured_by_advance_zero():
+    """advance(0) captures the step graph without stepping (a caller can
+    capture before other work is in flight) and leaves the state unchanged."""
+    e = T.Ensemble([N, N], [NU, 2 * NU], DT, T.SCHEME_RK4)
+    e.ic_random(0, 3)
+    e.ic_random(1, 4)
+    before = [e.download(r) for r in range(2)]
+    assert e.graph_nodes() == (0, 0)
+    e.advance(0)
+    nodes, kernels = e.graph_nodes()
+    assert kernels > 0 and nodes >= kernels
+    assert all(np.array_equal(e.download(r), before[r]) for r in range(2))
+    e.close()
 
 
 def test_batched_checker_matches_separate_checkers(data, trained):
@@ -114,11 +147,12 @@ def test_batched_checker_matches_separate_checkers(data, trained):
             checkers = [T.Ensemble([N] * (nsteps // every), [NU] * (nsteps // every), DT, T.SCHEME_RK4)]
         else:
             checkers = [T.Ensemble([N], [NU], DT, T.SCHEME_RK4) for _ in range(nsteps // every)]
+        anchor = T.Ensemble([N], [NU], DT, T.SCHEME_RK4)
         traj = torch.empty((nsteps + 1, 1, N, N), device="cuda")
         traj[0, 0].copy_(held[0, 1])
-        err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=0.0)
+        err, nf = net.rollout(1, traj, nsteps, every, checkers, STEPS, tol=0.0, anchor=anchor)
         outs.append((err, nf, traj.cpu().numpy()))
-        for c in checkers:
+        for c in checkers + [anchor]:
             c.close()
     (e0, n0, t0), (e1, n1, t1) = outs
     assert n0 == n1 == nsteps // every
