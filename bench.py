@@ -8,9 +8,11 @@ dealias + viscous term + the fused stage update) on one GPU, state resident in
 HBM.  value = Nx*Ny*steps / seconds.
 
 Prints one JSON line.  Also reports the roofline of the dominant kernel
-(algorithmic bytes / average HIP-event duration, against 8 TB/s HBM) and a CPU
-baseline: the reference's own derive2csh + FFTW on one host core (it is serial
-by construction), one RK4 step at the same size, timed in a child process.
+(algorithmic bytes / average HIP-event duration, against 8 TB/s HBM) and CPU
+baselines timed in child processes: the reference's own derive2csh + FFTW on
+one host core (it is serial by construction), three RK4 steps at the same
+size; and for the 3D line the reference's derive3com + ab3com with OpenMP and
+FFTW threads = the host's cores, three AB2 steps at 256^3.
 """
 import argparse
 import json
@@ -65,23 +67,35 @@ def model_2d(n, nsteps):
     return {0: 5 * S, 1: 5 * S, 2: 5 * S}, {0: "k_csh_yinv", 1: "k_csh_x", 2: "k_csh_yfwd"}, 60 * S
 
 
-def cpu_baseline(n):
+def cpu_baseline(case, n, steps):
+    """The reference on this host's cores in a child process (unlimited stack:
+    the reference keeps its work arrays there): 2D on one core (its csh path
+    is serial), 3D with OpenMP / FFTW threads = the cores this process may use."""
     def unlimited_stack():
         resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
-    env = dict(os.environ, OMP_NUM_THREADS="1")
-    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--n", str(n), "--steps", "1"],
-                       cwd=ROOT, capture_output=True, text=True, preexec_fn=unlimited_stack, env=env,
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    env = dict(os.environ, OMP_NUM_THREADS="1" if case == "2d" else str(cores))
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--case", case, "--n", str(n), "--steps",
+                        str(steps)], cwd=ROOT, capture_output=True, text=True, preexec_fn=unlimited_stack, env=env,
                        timeout=600)
     if r.returncode != 0:
         sys.stderr.write(r.stderr[-2000:])
         return None
     d = json.loads(r.stdout.strip().splitlines()[-1])
+    src = ("TARA Fortran + FFTW 3.3.9 built from the reference sources" if d["kind"] == "reference"
+           else "C restatement (not FFTW)")
+    if case == "2d":
+        sample = (f"{steps} RK4 steps (4 derive2csh calls + stage updates each) at {n}^2, "
+                  f"{d['seconds']:.1f} s on 1 host core, {src}")
+    else:
+        sample = (f"{steps} AB2 step(s) (derive3com + ab3com, nu = eta = 1e-3 via the one-constant patch) at "
+                  f"{n}^3 after one warm-up step, {d['seconds']:.1f} s on {d['cores']} threads, {src}")
     return {"value": d["value"], "unit": "grid-pt steps/s", "cores": d["cores"], "kind": d["kind"],
-            "sample": f"1 RK4 step (4 derive2csh calls + stage updates) at {n}^2, "
-                      f"{d['seconds']:.1f} s on 1 host core"
-                      + (", TARA Fortran + FFTW 3.3.9 built from the reference sources" if d["kind"] == "reference"
-                         else ", C restatement (not FFTW)")}
+            "sample": sample, "cpu_model": d["cpu_model"], "nproc": d["nproc"], "threads": d["cores"]}
 
 
 def bench_2d(n, steps, warmup, nu=1e-4, dt=2e-4):
@@ -317,8 +331,9 @@ def main():
     n = a.n
     nu, dt = 1e-4, 2e-4
 
-    # CPU baseline first, in a child process, before this process touches the GPU.
-    cpu = None if a.no_cpu_baseline else cpu_baseline(n)
+    # CPU baselines first, in child processes, before this process touches the GPU.
+    cpu = None if a.no_cpu_baseline else cpu_baseline("2d", n, 3)
+    cpu3 = None if (a.no_cpu_baseline or a.no_3d) else cpu_baseline("3d", 256, 3)
     if not a.no_fno:
         # torch (device memory of the surrogate leg) is loaded before the HIP
         # runtime starts in this process: loaded after, it finds no device.
@@ -358,6 +373,8 @@ def main():
                           "GBps": round(model[c] / (avg[c] * 1e-3) / 1e9, 1)} for c in avg}
     sec_1024 = bench_2d(1024, a.steps, a.warmup) if n != 1024 else None
     mhd = None if a.no_3d else bench_mhd(a.steps, a.warmup)
+    if mhd is not None:
+        mhd["cpu_baseline"] = cpu3
     ens = None if a.no_ensemble else bench_ensemble(min(a.steps, 20), a.warmup)
     fno = None if a.no_fno else bench_fno(a.fno_epochs)
     out = {

@@ -66,6 +66,16 @@ def build_shim():
     return out
 
 
+def build_gnucompat():
+    """libtara_gnucompat.so: GNU Fortran's RAND / SRAND for the kept mains (host C++)."""
+    out = os.path.join(HERE, "libtara_gnucompat.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", os.path.join(HERE, "shim", "gnu_compat.cpp"),
+          "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_fortran():
     """ISO_C_BINDING wrappers keeping the reference's subroutine names
     (libtara_fortran.so), and the Fortran test drivers under tests/fortran/."""
@@ -97,6 +107,7 @@ def build_fortran():
 def main():
     build_hip()
     build_shim()
+    build_gnucompat()
     build_fortran()
     print(LIB)
 

@@ -7,7 +7,7 @@ are produced by running its routines on the inputs below.  Run here (where
 /root/reference exists); the .npz files travel with the repository, the
 reference does not.
 
-    ulimit -s unlimited && python tests/golden/make_golden.py [core|fam2d|all]
+    ulimit -s unlimited && python tests/golden/make_golden.py [core|fam2d|long|all]
 
 (derive3com keeps ~130 automatic arrays on the stack, hence the stack limit.)
 """
@@ -183,8 +183,72 @@ def main():
     print("wrote", len(out), "arrays")
 
 
+def block(w, m):
+    """The modes kx <= m, |ky| <= m of a (Nh, N) half spectrum, ky ascending
+    from -m to m: shape (m + 1, 2 m + 1)."""
+    n = w.shape[1]
+    return np.asfortranarray(np.concatenate([w[:m + 1, n - m:], w[:m + 1, :m + 1]], axis=1))
+
+
+def main_long():
+    """Long-horizon and 3D fixtures (golden_long.npz):
+    * ab3com: one call of the reference ab3com (src/ab3com.f95:6-66) with
+      non-zero history at 8^3;
+    * d3mueta_16: the patched reference derive3com_mueta (diffusion mu on the
+      momentum, eta on the induction, oracle/Makefile) at mu = 0.02,
+      eta = 0.005 on the 16^3 state of golden.npz (d3com_in*);
+    * c2_512 / c2_1024: the RK4 harness over the reference derive2csh for
+      1000 steps at 512^2 and 1024^2, nu = 1e-4, dt = 2e-4, from the Philox
+      random-phase IC (oracle.ic_random, seed 1, k0 = 4, unit energy): the
+      final field (512^2) and, at 1024^2, the block kx <= 128, |ky| <= 128
+      plus the full field's L2 norm and the norm outside the block;
+    * mhd_32: the reference 3D AB2 loop (src/main3com.f95:536-617) over
+      derive3com_mueta + ab3com for 500 steps at 32^3, nu = eta = 1e-3,
+      dt = 5e-4, CS = 10, from oracle.ic_random3d(32, seed 1, k0 3, energies
+      1.0 and 0.5): the 7 live fields at step 500."""
+    out = {}
+    n = 8
+    rng = np.random.default_rng(11)
+
+    def rspec():
+        return O.r2c(np.asfortranarray(rng.standard_normal((n, n, n))))
+
+    st, fo, fn = [rspec() for _ in range(8)], [rspec() for _ in range(8)], [rspec() for _ in range(8)]
+    res = O.ref_ab3com(st, fo, fn, n, 1e-3)
+    for k in range(8):
+        out[f"ab3_st{k}"], out[f"ab3_fo{k}"], out[f"ab3_fn{k}"], out[f"ab3_out{k}"] = st[k], fo[k], fn[k], res[k]
+    st = mhd_state(7, 16)
+    f = O.ref_derive3com(st, 16, 2 * PI, 2 * PI, 2 * PI, cs=10.0, mu=0.02, eta=0.005)
+    for k in (0, 1, 2, 3, 5, 6, 7):
+        out[f"d3mueta_16_out{k}"] = f[k]
+    L = 2 * PI
+    ic = O.ic_random3d(32, seed=1, k0=3.0, e_kin=1.0, e_mag=0.5)
+    fin, _ = O.ref_mhd_ab2_steps(ic, 32, L, L, L, 1e-3, 1e-3, 5e-4, 500, cs=10.0)
+    for k in (0, 1, 2, 3, 5, 6, 7):
+        out[f"mhd_32_w500_{k}"] = fin[k]
+    print("3D done", flush=True)
+    for nn in (512, 1024):
+        w0 = O.ic_random(nn, nn, 1, 4.0, 1.0)
+        w = O.ref_rk4csh_steps(w0, nn, 1e-4, 2e-4, 1000)
+        if nn == 512:
+            out["c2_512_w1000"] = w
+        else:
+            b = block(w, 128)
+            out["c2_1024_block128"] = b
+            out["c2_1024_norm"] = np.array(np.linalg.norm(w))
+            inside = np.zeros(w.shape, bool)
+            inside[:129, :129] = True
+            inside[:129, nn - 128:] = True
+            out["c2_1024_norm_outside"] = np.array(np.linalg.norm(w[~inside]))
+        print(nn, "done", flush=True)
+    np.savez_compressed(os.path.join(HERE, "golden_long.npz"), **out)
+    print("wrote", len(out), "arrays to golden_long.npz")
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("long", "all"):
+        main_long()
     if what in ("core", "all"):
         main()
     if what in ("fam2d", "all"):
