@@ -57,6 +57,21 @@ int tara_fft_c2r_3d(int nx, int ny, int nz, const double* in, double* out);
  */
 int tara_derive2csh(int nx, int ny, int nh, double nu, const double* omegak, double* dt_omegak_new);
 
+/* No counterpart in the reference: tara_derive2csh for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  omegak and dt_omegak_new are DEVICE
+ * arrays in the reference layout; the work is enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) and the call returns without waiting.
+ * Every intermediate lives in the library's static 64 MiB per-device scratch,
+ * so grids are limited to nx, ny <= 1024 (larger ones: TARA_ERR_UNSUPPORTED,
+ * before anything is enqueued); no hipMalloc / hipFree is made.  The first
+ * call for a grid on a device writes that grid's tables into the scratch
+ * (waiting for `stream` first); later calls on the same grid only launch
+ * kernels.  Calls that overlap on one GPU race: they share its scratch.
+ * Results equal tara_derive2csh's bit for bit. */
+int tara_derive2csh_noalloc(int nx, int ny, int nh, double nu, const double* omegak, double* dt_omegak_new,
+                            void* stream);
+
 /* Replaces src/ab2csh.f95:3 ab2csh(Nx,Ny,Nh,nu,time,dt,omegak,omegak_new,
  * dt_omegak_old,dt_omegak_new): omegak_new = omegak + (3/2 f_n - 1/2 f_{n-1}) dt. */
 int tara_ab2csh(int nx, int ny, int nh, double dt, const double* omegak, double* omegak_new,

@@ -156,6 +156,8 @@ def lib():
         L.tara_mhd_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.tara_mhd_ic_abc.argtypes = [C.c_void_p] + [C.c_double] * 8
         L.tara_mhd_ic_random.argtypes = [C.c_void_p, C.c_uint] + [C.c_double] * 3
+        L.tara_derive2csh_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]
         L.tara_mhd_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
         _lib = L
     return _lib
@@ -215,6 +217,14 @@ def derive2csh(Nx, Ny, Nh, nu, omegak):
     out = np.zeros_like(w, order="F")
     _check(lib().tara_derive2csh(Nx, Ny, Nh, nu, _p(w), _p(out)))
     return out
+
+
+def derive2csh_noalloc(Nx, Ny, Nh, nu, omegak_dev, dt_omegak_new_dev, stream=0):
+    """derive2csh on device arrays (pointers to (Nh, Ny) complex128 half
+    spectra in the reference layout) without any device allocation, enqueued
+    on `stream` (a hipStream_t as an int; 0 = the null stream).  Grids up to
+    1024 x 1024; calls that overlap on one GPU race (they share its scratch)."""
+    _check(lib().tara_derive2csh_noalloc(Nx, Ny, Nh, nu, omegak_dev, dt_omegak_new_dev, stream or None))
 
 
 def ab2csh(Nx, Ny, Nh, dt, omegak, dt_omegak_old, dt_omegak_new):

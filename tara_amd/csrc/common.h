@@ -46,15 +46,24 @@ inline int ilog2(int n)
     return l;
 }
 
-// Device buffer owned by RAII (hipMalloc / hipFree).
+// Device buffer owned by RAII (hipMalloc / hipFree), or a view of memory
+// owned elsewhere (view(): never freed here).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool own = true;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     int alloc(size_t n);
+    void view(void* q, size_t n)
+    {
+        release();
+        p = q;
+        bytes = n;
+        own = false;
+    }
     void release();
     template <class T>
     T* as() const { return static_cast<T*>(p); }

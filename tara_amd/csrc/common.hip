@@ -34,9 +34,10 @@ int DevBuf::alloc(size_t n)
 
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    own = true;
 }
 
 int current_device()

@@ -90,6 +90,9 @@ struct CshPhys {
 // batch: number of independent runs of this grid the launches cover; every
 // per-run field (state, G0..G3, H) is packed run after run, bstride apart.
 int csh_plan_init(CshPlan& p, int nx, int ny, int batch = 1);
+// Host tables of a plan: derive2csh's wavenumbers kx (nh), ky (ny) and the
+// dealias bound imin (ny; modes i >= imin[j] drop the nonlinear term).
+int csh_host_tables(int nx, int ny, std::vector<double>& kx, std::vector<double>& ky, std::vector<int>& imin);
 // nsteps AB2 or RK4 steps (scheme: TARA_SCHEME_*) of derive2csh on the resident
 // state w, with each forward y pass fused with the inverse y pass that follows
 // it (k_csh_yfwd_inv): one stand-alone inverse y pass per call.  Timing classes

@@ -648,6 +648,34 @@ static int yfwd(const CshPlan& p, int mode, const YfwdArgs& a, hipStream_t st)
 }
 
 // ------------------------------------------------------------------ host side
+int csh_host_tables(int nx, int ny, std::vector<double>& kx, std::vector<double>& ky, std::vector<int>& imin)
+{
+    // wavenumber and dealias tables, with derive2csh's own expressions
+    const int nh = nx / 2 + 1;
+    kx.resize(nh);
+    ky.resize(ny);
+    imin.resize(ny);
+    const double pi_s = (double)3.1428f;  // real(8), parameter :: pi = 3.1428 (default-real literal)
+    const double L = 2.0 * pi_s;
+    for (int i = 0; i < nh; ++i) kx[i] = 2.0 * pi_s * (double)i / L;
+    for (int j = 0; j < ny; ++j) ky[j] = (j < ny / 2) ? 2.0 * pi_s * (double)j / L : 2.0 * pi_s * ((double)j - ny) / L;
+    for (int i = 0; i < nh; ++i)
+        TARA_CHECK(kx[i] == (double)i, TARA_ERR_UNSUPPORTED, "derive2csh kx(%d) is not an integer", i);
+    for (int j = 0; j < ny; ++j)
+        TARA_CHECK(ky[j] == (double)(j < ny / 2 ? j : j - ny), TARA_ERR_UNSUPPORTED, "derive2csh ky(%d) not an integer", j);
+    const double cut = ((double)(nx + ny) / 2.0) / 3.0;
+    for (int j = 0; j < ny; ++j) {
+        int m = nh;
+        for (int i = 0; i < nh; ++i)
+            if (std::sqrt(kx[i] * kx[i] + ky[j] * ky[j]) >= cut) {
+                m = i;
+                break;
+            }
+        imin[j] = m;  // kx increases with i, so every i >= m is cut as well
+    }
+    return TARA_OK;
+}
+
 int csh_plan_init(CshPlan& p, int nx, int ny, int batch)
 {
     TARA_CHECK(batch >= 1 && batch <= 65535, TARA_ERR_ARG, "batch %d out of range", batch);
@@ -670,27 +698,9 @@ int csh_plan_init(CshPlan& p, int nx, int ny, int batch)
     if ((rc = p.h.alloc(batch * ns * sizeof(double2)))) return rc;
     if ((rc = p.bad.alloc(batch * sizeof(int)))) return rc;
     TARA_HIP(hipMemset(p.bad.p, 0, batch * sizeof(int)));
-    // wavenumber and dealias tables, with derive2csh's own expressions
-    std::vector<double> kx(p.nh), ky(ny);
-    std::vector<int> imin(ny);
-    const double pi_s = (double)3.1428f;  // real(8), parameter :: pi = 3.1428 (default-real literal)
-    const double L = 2.0 * pi_s;
-    for (int i = 0; i < p.nh; ++i) kx[i] = 2.0 * pi_s * (double)i / L;
-    for (int j = 0; j < ny; ++j) ky[j] = (j < ny / 2) ? 2.0 * pi_s * (double)j / L : 2.0 * pi_s * ((double)j - ny) / L;
-    for (int i = 0; i < p.nh; ++i)
-        TARA_CHECK(kx[i] == (double)i, TARA_ERR_UNSUPPORTED, "derive2csh kx(%d) is not an integer", i);
-    for (int j = 0; j < ny; ++j)
-        TARA_CHECK(ky[j] == (double)(j < ny / 2 ? j : j - ny), TARA_ERR_UNSUPPORTED, "derive2csh ky(%d) not an integer", j);
-    const double cut = ((double)(nx + ny) / 2.0) / 3.0;
-    for (int j = 0; j < ny; ++j) {
-        int m = p.nh;
-        for (int i = 0; i < p.nh; ++i)
-            if (std::sqrt(kx[i] * kx[i] + ky[j] * ky[j]) >= cut) {
-                m = i;
-                break;
-            }
-        imin[j] = m;  // kx increases with i, so every i >= m is cut as well
-    }
+    std::vector<double> kx, ky;
+    std::vector<int> imin;
+    if ((rc = csh_host_tables(nx, ny, kx, ky, imin))) return rc;
     if ((rc = p.kx.alloc(p.nh * sizeof(double)))) return rc;
     if ((rc = p.ky.alloc(ny * sizeof(double)))) return rc;
     if ((rc = p.imin.alloc(ny * sizeof(int)))) return rc;
