@@ -330,6 +330,19 @@ int tara_mhd_ic_abc(tara_mhd* ctx, double rho0, double u0, double b0, double p0,
  * raw vector is projected with its own wavevector.  No counterpart in the
  * reference (its 3D initial state is the ABC flow of tara_mhd_ic_abc). */
 int tara_mhd_ic_random(tara_mhd* ctx, unsigned seed, double k0, double e_kin, double e_mag);
+/* The reference driver's per-step diagnostics (src/main3com.f95:619-1071,
+ * the columns after `time` of its Energy.dat row, :1070) on the resident
+ * state, computed on the device: out[0..9] = Pressure <CS^2 rho>, Energy
+ * <|u|^2>/2, T_Energy <E>, B_Field <|B|^2>/(2 mu_0) (mu_0 = 1), C1 <|omega|>,
+ * C2 <|omega|^2>, Hf <u.omega>, Hm <A.B>, HT <(u+A).(omega+B)>, div_B_Tot
+ * <div B>, with u = rho u/rho, A = curl B/k^2 and omega formed as that loop
+ * forms them (its omega_y is i kx uz - i kz ux).  tara_mhd_record_energy
+ * advances nrec x every steps and writes the ten values after every `every`
+ * steps into rows[nrec][10] (one device-to-host copy at the end).  Both are
+ * synchronous; the first call allocates the work space (15 real and 11
+ * spectral fields). */
+int tara_mhd_energy(tara_mhd* ctx, double out[10]);
+int tara_mhd_record_energy(tara_mhd* ctx, int nrec, int every, double* rows);
 /* Kernel timing as for tara_csh: classes 0 = inverse z+y passes, 1 = x pass,
  * 2 = forward y pass, 3 = forward z pass + assembly + update. */
 int tara_mhd_set_timing(tara_mhd* ctx, int on);

@@ -156,6 +156,8 @@ def lib():
         L.tara_mhd_set_timing.argtypes = [C.c_void_p, C.c_int]
         L.tara_mhd_ic_abc.argtypes = [C.c_void_p] + [C.c_double] * 8
         L.tara_mhd_ic_random.argtypes = [C.c_void_p, C.c_uint] + [C.c_double] * 3
+        L.tara_mhd_energy.argtypes = [C.c_void_p, _dp]
+        L.tara_mhd_record_energy.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         L.tara_derive2csh_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
         L.tara_mhd_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), _dp]
@@ -698,6 +700,20 @@ class MhdSolver:
     def ic_random(self, seed=1, k0=3.0, e_kin=1.0, e_mag=0.5):
         """Random-phase solenoidal u and B on the device (tara_mhd_ic_random)."""
         _check(lib().tara_mhd_ic_random(self._h, seed, k0, e_kin, e_mag))
+
+    ENERGY_COLUMNS = ("Pressure", "Energy", "T_Energy", "B_Field", "C1", "C2", "Hf", "Hm", "HT", "div_B_Tot")
+
+    def energy(self):
+        """The ten Energy.dat averages of src/main3com.f95:1070 for the current state."""
+        out = np.zeros(10)
+        _check(lib().tara_mhd_energy(self._h, _p(out)))
+        return out
+
+    def record_energy(self, nrec, every=1):
+        """Advances nrec x every steps; returns the averages after every `every` steps, (nrec, 10)."""
+        rows = np.zeros((nrec, 10))
+        _check(lib().tara_mhd_record_energy(self._h, nrec, every, _p(rows)))
+        return rows
 
     def sync(self):
         _check(lib().tara_mhd_sync(self._h))

@@ -128,6 +128,10 @@ struct tara_mhd {
     hipStream_t s = nullptr;
     double dt = 0;
     std::unique_ptr<EventTimer> tm;
+    // diagnostics work space (allocated on the first tara_mhd_energy call):
+    // spectral A (3), div B, u (3), omega (3) and one copy buffer; 15 real
+    // fields; per-block partial sums
+    DevBuf dspec[11], dreal[15], dpart;
 };
 
 extern "C" int tara_mhd_create(int nx, int ny, int nz, const tara_mhd_params* prm, tara_mhd** out)
@@ -453,6 +457,236 @@ extern "C" int tara_mhd_ic_random(tara_mhd* c, unsigned seed, double k0, double 
     for (int f = 0; f < 7; ++f) TARA_HIP(hipMemsetAsync(c->hist[f].p, 0, c->hist[f].bytes, s));
     TARA_HIP(hipMemsetAsync(c->plan.bad.p, 0, sizeof(int), s));
     TARA_HIP(hipStreamSynchronize(s));
+    return TARA_OK;
+}
+
+// ------------------------------------------------------------ diagnostics
+// The per-step diagnostics of the reference's 3D driver, src/main3com.f95:
+// 619-1071, on the resident state: div B, j = curl B and A = j/k^2 in spectral
+// space (:621-686), the real fields (:688-835), u = rho u/rho and its
+// transform (:837-880), omega = curl u as that loop writes it, including its
+// omega_y = i kx uz - i kz ux (:882-975), and the volume averages of
+// :1024-1071, written per step to Energy.dat.
+namespace {
+
+constexpr int kDiagBlocks = 1024;  // grid of the reduction (its partial sums are added in a fixed order)
+constexpr int kDiagSums = 10;
+
+// i k s (the reference's (0,1)*k*s: (0, k) times s)
+__device__ __forceinline__ double2 dg_ik(double k, double2 s) { return make_double2(-(k * s.y), k * s.x); }
+__device__ __forceinline__ double2 dg_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// div B, A = (i k x B)/k^2 (A = j at k = 0) from the B spectra (src/main3com.f95:623-686)
+__global__ void k_diag_spec_b(const double2* bx, const double2* by, const double2* bz, const double* kx,
+                              const double* ky, const double* kz, int ld, int ny, int nz, double2* ax, double2* ay,
+                              double2* az, double2* divb)
+{
+    const size_t n = (size_t)ld * ny * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % ld), j = (int)((q / ld) % ny), k = (int)(q / ((size_t)ld * ny));
+        const double kxv = kx[i], kyv = ky[j], kzv = kz[k];
+        const double2 x = bx[q], y = by[q], z = bz[q];
+        const double2 d1 = dg_ik(kxv, x), d2 = dg_ik(kyv, y), d3 = dg_ik(kzv, z);
+        divb[q] = make_double2((d1.x + d2.x) + d3.x, (d1.y + d2.y) + d3.y);
+        const double2 jx = dg_sub(dg_ik(kyv, z), dg_ik(kzv, y));
+        const double2 jy = dg_sub(dg_ik(kzv, x), dg_ik(kxv, z));
+        const double2 jz = dg_sub(dg_ik(kxv, y), dg_ik(kyv, x));
+        if (q == 0) {
+            ax[q] = jx;
+            ay[q] = jy;
+            az[q] = jz;
+        } else {
+            const double k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+            // padding columns (kx = 0, k2 = 0 only at q = 0 otherwise) stay zero
+            const double2 zero = make_double2(0.0, 0.0);
+            ax[q] = k2 != 0.0 ? make_double2(jx.x / k2, jx.y / k2) : zero;
+            ay[q] = k2 != 0.0 ? make_double2(jy.x / k2, jy.y / k2) : zero;
+            az[q] = k2 != 0.0 ? make_double2(jz.x / k2, jz.y / k2) : zero;
+        }
+    }
+}
+
+// u = rho u / rho in real space (src/main3com.f95:804-806), in place over rho u
+__global__ void k_diag_u(const double* rho, double* ux, double* uy, double* uz, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const double r = rho[q];
+        ux[q] = ux[q] / r;
+        uy[q] = uy[q] / r;
+        uz[q] = uz[q] / r;
+    }
+}
+
+// omega from the u spectra as src/main3com.f95:889-975 writes it
+__global__ void k_diag_spec_w(const double2* ux, const double2* uy, const double2* uz, const double* kx,
+                              const double* ky, const double* kz, int ld, int ny, int nz, double2* wx, double2* wy,
+                              double2* wz)
+{
+    const size_t n = (size_t)ld * ny * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % ld), j = (int)((q / ld) % ny), k = (int)(q / ((size_t)ld * ny));
+        const double kxv = kx[i], kyv = ky[j], kzv = kz[k];
+        const double2 x = ux[q], y = uy[q], z = uz[q];
+        wx[q] = dg_sub(dg_ik(kyv, z), dg_ik(kzv, y));
+        wy[q] = dg_sub(dg_ik(kxv, z), dg_ik(kzv, x));
+        wz[q] = dg_sub(dg_ik(kxv, y), dg_ik(kyv, x));
+    }
+}
+
+struct DiagFields {
+    const double *rho, *ux, *uy, *uz, *e, *bx, *by, *bz, *ax, *ay, *az, *wx, *wy, *wz, *divb;
+};
+
+// Per-point terms of the averages of src/main3com.f95:1034-1064 (each divided
+// by Nx Ny Nz there; here once, after the sum): P = CS^2 rho, |u|^2/2, E,
+// |B|^2/(2 mu_0), |omega|, |omega|^2, u . omega, A . B, (u + A).(omega + B),
+// div B; per-block sums into partial[block][10].
+__global__ __launch_bounds__(256) void k_diag_reduce(DiagFields f, size_t n, double cs2, double mu0, double* partial)
+{
+    __shared__ double red[kDiagSums][256];
+    double acc[kDiagSums] = {0};
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const double ux = f.ux[q], uy = f.uy[q], uz = f.uz[q];
+        const double bx = f.bx[q], by = f.by[q], bz = f.bz[q];
+        const double ax = f.ax[q], ay = f.ay[q], az = f.az[q];
+        const double wx = f.wx[q], wy = f.wy[q], wz = f.wz[q];
+        const double w2 = wx * wx + wy * wy + wz * wz;
+        const double wn = sqrt(w2);
+        acc[0] += cs2 * f.rho[q];
+        acc[1] += (ux * ux + uy * uy + uz * uz) / 2.0;
+        acc[2] += f.e[q];
+        acc[3] += (bx * bx + by * by + bz * bz) / (2.0 * mu0);
+        acc[4] += wn;
+        acc[5] += wn * wn;
+        acc[6] += ux * wx + uy * wy + uz * wz;
+        acc[7] += ax * bx + ay * by + az * bz;
+        acc[8] += (ux + ax) * (wx + bx) + (uy + ay) * (wy + by) + (uz + az) * (wz + bz);
+        acc[9] += f.divb[q];
+    }
+#pragma unroll
+    for (int c = 0; c < kDiagSums; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int c = 0; c < kDiagSums; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + h];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < kDiagSums) partial[(size_t)blockIdx.x * kDiagSums + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[c] = sum over blocks of partial[.][c] / nn, in block order
+__global__ void k_diag_final(const double* partial, int nblocks, double nn, double* out)
+{
+    const int c = threadIdx.x;
+    if (c >= kDiagSums) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * kDiagSums + c];
+    out[c] = s / nn;
+}
+
+unsigned diag_grid(size_t n)
+{
+    const size_t b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b ? b : 1));
+}
+
+// spectral (padded, destroyed) -> real (nx, ny, nz) / (Nx Ny Nz)
+int diag_c2r(const MhdPlan& p, double2* spec, double* real, hipStream_t s)
+{
+    int rc;
+    if ((rc = col_fft(spec, p.nz, p.ld, (long)p.ld * p.ny, p.ld, p.ny, +1, s))) return rc;
+    if ((rc = col_fft(spec, p.ny, p.ld, p.ld, (long)p.ld * p.ny, p.nz, +1, s))) return rc;
+    return row_c2r(spec, real, p.nx, p.ld, p.ny * p.nz, 1.0 / ((double)p.nx * p.ny * p.nz), s);
+}
+
+// real (nx, ny, nz) -> spectral (padded)
+int diag_r2c(const MhdPlan& p, const double* real, double2* spec, hipStream_t s)
+{
+    int rc;
+    TARA_HIP(hipMemsetAsync(spec, 0, p.fsz * sizeof(double2), s));
+    if ((rc = row_r2c(real, spec, p.nx, p.ld, p.ny * p.nz, s))) return rc;
+    if ((rc = col_fft(spec, p.ny, p.ld, p.ld, (long)p.ld * p.ny, p.nz, -1, s))) return rc;
+    return col_fft(spec, p.nz, p.ld, (long)p.ld * p.ny, p.ld, p.ny, -1, s);
+}
+
+// Enqueues the diagnostics of the current state; the 10 averages land in
+// device memory at out (ordered on the context's stream).
+int mhd_diag_enqueue(tara_mhd* c, double* out)
+{
+    const MhdPlan& p = c->plan;
+    hipStream_t s = c->s;
+    int rc;
+    if (!c->dpart.p) {
+        for (auto& b : c->dspec)
+            if ((rc = b.alloc(p.fsz * sizeof(double2)))) return rc;
+        for (auto& b : c->dreal)
+            if ((rc = b.alloc((size_t)p.nx * p.ny * p.nz * sizeof(double)))) return rc;
+        if ((rc = c->dpart.alloc((size_t)(kDiagBlocks + 1) * kDiagSums * sizeof(double)))) return rc;
+    }
+    const size_t nsp = p.fsz, nr = (size_t)p.nx * p.ny * p.nz;
+    double2 *A[3] = {c->dspec[0].as<double2>(), c->dspec[1].as<double2>(), c->dspec[2].as<double2>()};
+    double2* divb = c->dspec[3].as<double2>();
+    double2 *U[3] = {c->dspec[4].as<double2>(), c->dspec[5].as<double2>(), c->dspec[6].as<double2>()};
+    double2 *W[3] = {c->dspec[7].as<double2>(), c->dspec[8].as<double2>(), c->dspec[9].as<double2>()};
+    double2* tmp = c->dspec[10].as<double2>();
+    double* R[15];
+    for (int f = 0; f < 15; ++f) R[f] = c->dreal[f].as<double>();
+    // real fields: 0 rho, 1-3 rho u (then u), 4 E, 5-7 B, 8-10 A, 11-13 omega, 14 div B
+    hipLaunchKernelGGL(k_diag_spec_b, dim3(diag_grid(nsp)), dim3(256), 0, s, c->st[5].as<double2>(),
+                       c->st[6].as<double2>(), c->st[7].as<double2>(), p.kx.as<double>(), p.ky.as<double>(),
+                       p.kz.as<double>(), p.ld, p.ny, p.nz, A[0], A[1], A[2], divb);
+    TARA_HIP(hipGetLastError());
+    for (int f = 0; f < 8; ++f) {  // rho, rho u, E, B (the state's order)
+        TARA_HIP(hipMemcpyAsync(tmp, c->st[f].p, nsp * sizeof(double2), hipMemcpyDeviceToDevice, s));
+        if ((rc = diag_c2r(p, tmp, R[f], s))) return rc;
+    }
+    for (int f = 0; f < 3; ++f)
+        if ((rc = diag_c2r(p, A[f], R[8 + f], s))) return rc;
+    if ((rc = diag_c2r(p, divb, R[14], s))) return rc;
+    hipLaunchKernelGGL(k_diag_u, dim3(diag_grid(nr)), dim3(256), 0, s, R[0], R[1], R[2], R[3], nr);
+    TARA_HIP(hipGetLastError());
+    for (int f = 0; f < 3; ++f)
+        if ((rc = diag_r2c(p, R[1 + f], U[f], s))) return rc;
+    hipLaunchKernelGGL(k_diag_spec_w, dim3(diag_grid(nsp)), dim3(256), 0, s, U[0], U[1], U[2], p.kx.as<double>(),
+                       p.ky.as<double>(), p.kz.as<double>(), p.ld, p.ny, p.nz, W[0], W[1], W[2]);
+    TARA_HIP(hipGetLastError());
+    for (int f = 0; f < 3; ++f)
+        if ((rc = diag_c2r(p, W[f], R[11 + f], s))) return rc;
+    DiagFields df{R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8], R[9], R[10], R[11], R[12], R[13], R[14]};
+    double* part = c->dpart.as<double>();
+    hipLaunchKernelGGL(k_diag_reduce, dim3(kDiagBlocks), dim3(256), 0, s, df, nr, p.cs * p.cs, 1.0, part);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, part, kDiagBlocks, (double)nr, out);
+    TARA_HIP(hipGetLastError());
+    return TARA_OK;
+}
+
+}  // namespace
+
+extern "C" int tara_mhd_energy(tara_mhd* c, double* out)
+{
+    TARA_CHECK(c && out, TARA_ERR_ARG, "null argument");
+    DevBuf row;
+    int rc;
+    if ((rc = row.alloc(kDiagSums * sizeof(double))) || (rc = mhd_diag_enqueue(c, row.as<double>()))) return rc;
+    TARA_HIP(hipMemcpyAsync(out, row.p, kDiagSums * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    TARA_HIP(hipStreamSynchronize(c->s));
+    return TARA_OK;
+}
+
+extern "C" int tara_mhd_record_energy(tara_mhd* c, int nrec, int every, double* rows)
+{
+    TARA_CHECK(c && rows && nrec >= 0 && every >= 1, TARA_ERR_ARG, "bad argument");
+    if (nrec == 0) return TARA_OK;
+    DevBuf dev;
+    int rc;
+    if ((rc = dev.alloc((size_t)nrec * kDiagSums * sizeof(double)))) return rc;
+    for (int r = 0; r < nrec; ++r) {
+        if ((rc = tara_mhd_advance(c, every))) return rc;
+        if ((rc = mhd_diag_enqueue(c, dev.as<double>() + (size_t)r * kDiagSums))) return rc;
+    }
+    TARA_HIP(hipMemcpyAsync(rows, dev.p, (size_t)nrec * kDiagSums * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    TARA_HIP(hipStreamSynchronize(c->s));
     return TARA_OK;
 }
 

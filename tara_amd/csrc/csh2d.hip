@@ -276,6 +276,14 @@ __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
         const int m = rf_first_idx<M, EM>(tm, e);
         z[e] = make_double2(xr[2 * m], xr[2 * m + 1]);
     }
+    // The unpacking roots of this thread's kx = t + u T, fetched now, before
+    // any store (loads and stores share one in-order counter: a root loaded
+    // between the stores would also wait for the stores before it).  4096^2
+    // x pass 0.274 against 0.278 ms.
+    constexpr int NK = M / T;
+    double2 wk[NK];
+#pragma unroll
+    for (int u = 0; u < NK; ++u) wk[u] = rctw[t + u * T];
     __syncthreads();
     constexpr int HALF = SPLIT ? RF<M, EM>::LDS / 2 : RF<M, EM>::LDS;  // upper half, complex elements
     double2* zl = xl + (t < S::TM ? 0 : HALF);
@@ -285,11 +293,13 @@ __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
         for (int e = 0; e < EH; ++e) xl[rf_pad(rf_final_idx<M, EM>(tm, e))] = z[e];
     }
     __syncthreads();
-    for (int k = t; k < M; k += T) {  // consecutive t -> consecutive kx
+#pragma unroll
+    for (int u = 0; u < NK; ++u) {  // consecutive t -> consecutive kx
+        const int k = t + u * T;
         const double2 zk = xl[rf_pad(k)];
         const double2 zmk = xl[rf_pad(k == 0 ? 0 : M - k)];
         double2 xm;
-        h[blk8(k, j, M + 1)] = r2c_unpack(zk, zmk, k, rctw, &xm);
+        h[blk8(k, j, M + 1)] = r2c_unpack_w(zk, zmk, k, wk[u], &xm);
         if (k == 0) h[blk8(M, j, M + 1)] = xm;
     }
 }

@@ -61,8 +61,11 @@ __device__ __forceinline__ double2 c2r_pack(double2 xk, double2 xmk, int k, cons
 // r2c unpacking along x: Z = forward length-M FFT of z[m] = x[2m] + i x[2m+1];
 // X[k] = 1/2 (Z[k] + conj Z[M-k]) - i/2 e^{-2 pi i k/Nx} (Z[k] - conj Z[M-k]),
 // with Z[M] == Z[0].  For k = 0 also returns X[M] = Re Z0 - Im Z0 in *xm.
-__device__ __forceinline__ double2 r2c_unpack(double2 zk, double2 zmk, int k, const double2* __restrict__ rctw,
-                                              double2* xm)
+// The same with the root e^{-2 pi i k/Nx} passed in (wk = rctw[k]), so that a
+// caller can fetch its roots before it issues stores: on gfx950 loads and
+// stores share one in-order counter, and waiting for a load issued after a
+// store also waits for the store.
+__device__ __forceinline__ double2 r2c_unpack_w(double2 zk, double2 zmk, int k, double2 wk, double2* xm)
 {
     if (k == 0) {
         *xm = make_double2(zk.x - zk.y, 0.0);
@@ -70,8 +73,14 @@ __device__ __forceinline__ double2 r2c_unpack(double2 zk, double2 zmk, int k, co
     }
     const double2 e = make_double2(0.5 * (zk.x + zmk.x), 0.5 * (zk.y - zmk.y));
     const double2 d = make_double2(0.5 * (zk.x - zmk.x), 0.5 * (zk.y + zmk.y));
-    const double2 t = cmul(rctw[k], d);                            // e^{-2 pi i k/Nx} d
+    const double2 t = cmul(wk, d);                                 // e^{-2 pi i k/Nx} d
     return make_double2(e.x + t.y, e.y - t.x);                     // e - i t
+}
+
+__device__ __forceinline__ double2 r2c_unpack(double2 zk, double2 zmk, int k, const double2* __restrict__ rctw,
+                                              double2* xm)
+{
+    return r2c_unpack_w(zk, zmk, k, k == 0 ? make_double2(1.0, 0.0) : rctw[k], xm);
 }
 
 }  // namespace tara

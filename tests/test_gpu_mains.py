@@ -100,3 +100,29 @@ def test_reference_main3com_through_tara_vs_reference_program(tmp_path, n, steps
     # equation keeps it solenoidal): below 1e-12 of B^2's scale in both
     bscale = np.max(np.abs(want[:, 4]))
     assert np.max(np.abs(got[:, -1])) < 1e-12 * bscale and np.max(np.abs(want[:, -1])) < 1e-12 * bscale
+
+
+@pytest.mark.parametrize("n,steps", [(16, 20)])
+def test_device_diagnostics_vs_reference_program(tmp_path, n, steps):
+    """The device-resident path with the device diagnostics: the reference's
+    ABC-flow initial state (tara_mhd_ic_abc with main3com's parameters,
+    src/main3com.f95:228-324), AB2 steps with the reference's 1/450 diffusion
+    and CS = 1/M = 10, and the ten Energy.dat averages after every step
+    (tara_mhd_record_energy) against the reference program's Energy.dat."""
+    import tara_amd as T
+
+    print(f"reference program {n}^3 x {steps} (CPU)", flush=True)
+    want = _run(TARA_REF, tmp_path / "ref", n, steps)
+    L = 2.0 * np.pi
+    s = T.MhdSolver(n, n, n, T.MhdParams(L, L, L, 1.0 / 450.0, 1.0 / 450.0, 10.0, 1e-3))
+    s.ic_abc(rho0=1.0, u0=0.1, b0=0.1, p0=1.0)
+    print(f"device stepping + diagnostics {n}^3 x {steps}", flush=True)
+    got = s.record_energy(steps, 1)
+    s.close()
+    for c, name in enumerate(COLUMNS[1:-1]):
+        scale = np.max(np.abs(want[:, c + 1]))
+        err = np.max(np.abs(got[:, c] - want[:, c + 1])) / scale
+        print(f"  {name}: max rel diff {err:.2e}", flush=True)
+        assert err < 1e-10, (name, err)
+    bscale = np.max(np.abs(want[:, 4]))
+    assert np.max(np.abs(got[:, -1])) < 1e-12 * bscale

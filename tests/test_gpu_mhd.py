@@ -149,3 +149,34 @@ def test_mhd_ic_abc_vs_numpy(n):
         # reference layout: (Nh, Ny, Nz) half spectrum over the first (x) index
         want = np.fft.rfftn(r.transpose(2, 1, 0), axes=(0, 1, 2)).transpose(2, 1, 0)
         assert rel_l2(got[f], want) < 1e-13, f
+
+
+def test_mhd_energy_closed_form_abc():
+    """The device diagnostics of the ABC-flow state (rho = 1, u = u0 (A sin z +
+    C cos y, B sin x + A cos z, C sin y + B cos x), uniform B = b0 on each
+    axis, E = p0 + u^2/2 + B^2/2) against their closed forms: <CS^2 rho> =
+    CS^2, <|u|^2>/2 = u0^2 (A^2+B^2+C^2)/2, <E> = p0 + that + 3 b0^2/2,
+    <|B|^2>/2 = 3 b0^2/2, A = curl B/k^2 = 0 (uniform B), div B = 0; the
+    flow is Beltrami (curl u = u), and the driver's omega has its y component
+    negated (src/main3com.f95:890), so |omega| = |u| pointwise: <|omega|^2> =
+    u0^2 (A^2+B^2+C^2), u.omega = ux^2 - uy^2 + uz^2, whose mean is
+    u0^2 (A^2 + C^2 - B^2 - A^2 + C^2 + B^2)/2 = u0^2 C^2; HT = Hf (A = 0,
+    and <u.B> = <omega.B> = 0 for a uniform B and zero-mean u)."""
+    n, u0, b0, p0 = 32, 0.1, 0.1, 1.0
+    a, b, c = 1.0, 0.7, 0.4
+    s = T.MhdSolver(n, n, n, params(1.0 / 450, 1.0 / 450))
+    s.ic_abc(1.0, u0, b0, p0, a, b, c, 1.0)
+    got = s.energy()
+    s.close()
+    ke = u0 ** 2 * (a * a + b * b + c * c) / 2
+    want = {0: 100.0, 1: ke, 2: p0 + ke + 1.5 * b0 ** 2, 3: 1.5 * b0 ** 2, 5: 2 * ke, 6: u0 ** 2 * c * c,
+            7: 0.0, 8: u0 ** 2 * c * c, 9: 0.0}
+    for k, v in want.items():
+        assert abs(got[k] - v) < 1e-13 * max(1.0, abs(v)), (k, got[k], v)
+    # <|omega|> = <|u|>: the mean speed of the ABC field, by quadrature on a finer grid
+    m = 96
+    x = np.arange(m) * 2 * np.pi / m
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij", sparse=True)
+    speed = u0 * np.sqrt((a * np.sin(Z) + c * np.cos(Y)) ** 2 + (b * np.sin(X) + a * np.cos(Z)) ** 2
+                         + (c * np.sin(Y) + b * np.cos(X)) ** 2)
+    assert abs(got[4] - speed.mean()) < 1e-3 * got[4]
