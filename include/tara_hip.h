@@ -119,6 +119,20 @@ int tara_csh_set_xshape(tara_csh* ctx, int shape);
  * (frames: nframes consecutive (Nh, Ny) half spectra). */
 int tara_csh_fields(tara_csh* ctx, double* omega, double* ux, double* uy);
 int tara_csh_record(tara_csh* ctx, int nframes, int every, double* frames);
+/* The driver's tracer particles (src/main2csh.f95:187-265) on the device:
+ * nsteps times, one step of the state followed by the particle update with
+ * the new state's velocity (ux = i ky psi, uy = -i kx psi with the driver's
+ * kx = 2 pi (i-1)/lx, ky likewise, psi = omega/k^2; c2r divided by Nx*Ny),
+ * bilinear interpolation as the driver writes it, x <- x + u dt and the wrap
+ * into (0, lx] x (0, ly].  xp, yp: host arrays of np positions in
+ * [0, lx] x [0, ly], updated in place; lx, ly: the driver's box (the input
+ * values times pi); dt: the driver's step.  periodic == 0 reads the corners
+ * as the driver addresses its column-major arrays (the last column's right
+ * neighbour is the next row's first element; reads past the end, undefined
+ * there, wrap to the start); periodic != 0 takes every corner's periodic
+ * image.  Synchronous; TARA_ERR_NONFINITE if a position became non-finite. */
+int tara_csh_advance_tracers(tara_csh* ctx, int nsteps, double dt, double lx, double ly, int periodic, int np,
+                             double* xp, double* yp);
 /* Initial conditions generated on the device into the context's state (the AB2
  * history is zeroed).  tara_csh_ic_shear: the shear layer of
  * src/main2csh.f95:84-110 (box and d as input.ini gives them, multiplied by pi

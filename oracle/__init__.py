@@ -168,6 +168,73 @@ def ic2csh(nx, ny, lx_in=2.0, ly_in=2.0, w0=25.0, m0=2.0, d_in=3.0):
     return omega, wk
 
 
+# --------------------------------------------------------- tracer particles
+def csh_velocity(w, nx, lx, ly):
+    """ux, uy of main2csh's per-step work (src/main2csh.f95:128-167): the
+    driver's wavenumbers kx = 2 pi (i-1)/Lx, ky = 2 pi (j-1)/Ly for
+    j <= Ny/2 else 2 pi ((j-1)-Ny)/Ly, psi = omega/(kx^2+ky^2) with
+    psi(1,1) = 0, ukx = +i ky psi, uky = -i kx psi, unnormalised c2r
+    (FFTW semantics, as c2r here) divided by Nx*Ny."""
+    w = np.asfortranarray(w, dtype=np.complex128)
+    nh, ny = w.shape
+    kx = (2.0 * np.pi * np.arange(nh, dtype=float) / lx)[:, None]
+    j = np.arange(ny)
+    ky = (2.0 * np.pi * np.where(j < ny // 2, j, j - ny).astype(float) / ly)[None, :]
+    k2 = kx * kx + ky * ky
+    k2[0, 0] = 1.0
+    psi = (w.real / k2) + 1j * (w.imag / k2)
+    psi[0, 0] = 0.0
+    ukx = (-(ky * psi.imag)) + 1j * (ky * psi.real)
+    uky = (kx * psi.imag) + 1j * (-(kx * psi.real))
+    n = float(nx) * float(ny)
+    return c2r(ukx, nx) / n, c2r(uky, nx) / n
+
+
+def tracer_step(ux, uy, xp, yp, lx, ly, dt, periodic=False):
+    """One step of the tracer particles (src/main2csh.f95:187-265), returning
+    the new (xp, yp).  Cell m = aint(xp/dx) (0-based here), bilinear
+    interpolation in the driver's expression (h = 1/(dx dy), x(m+1) = Lx in
+    the last column, y(n+1) = Ly in the last row), x <- x + v dt, then
+    x <- x - int(x/Lx) Lx and + Lx if <= 0.  Corners: with periodic=False as
+    the driver addresses its column-major arrays (flat index m + n Nx, +1 for
+    the right neighbour, +Nx for the upper one), reads past the end wrapping
+    to the start (the driver's are undefined there); with periodic=True the
+    periodic images.  A cell index at or past Nx (xp = Lx) is taken as the
+    last cell."""
+    ux = np.asfortranarray(ux)
+    uy = np.asfortranarray(uy)
+    nx, ny = ux.shape
+    dx, dy = lx / nx, ly / ny
+    h = 1.0 / (dx * dy)
+    xp = np.asarray(xp, dtype=float)
+    yp = np.asarray(yp, dtype=float)
+    m = np.clip(np.trunc(xp / dx).astype(np.int64), 0, nx - 1)
+    n = np.clip(np.trunc(yp / dy).astype(np.int64), 0, ny - 1)
+    xm, ym = m * dx, n * dy
+    xm1 = np.where(m + 1 < nx, (m + 1) * dx, lx)
+    yn1 = np.where(n + 1 < ny, (n + 1) * dy, ly)
+    total = nx * ny
+    a = m + n * nx
+    if periodic:
+        m1, n1 = (m + 1) % nx, (n + 1) % ny
+        b, c, d = m1 + n * nx, m + n1 * nx, m1 + n1 * nx
+    else:
+        b, c, d = (a + 1) % total, (a + nx) % total, (a + nx + 1) % total
+    fx, fy = ux.ravel(order="F"), uy.ravel(order="F")
+
+    def interp(f):
+        return (f[a] * (xm1 - xp) * (yn1 - yp) * h + f[b] * (xp - xm) * (yn1 - yp) * h
+                + f[c] * (xm1 - xp) * (yp - ym) * h + f[d] * (xp - xm) * (yp - ym) * h)
+
+    x = xp + interp(fx) * dt
+    y = yp + interp(fy) * dt
+    x = x - np.trunc(x / lx) * lx
+    x = np.where(x <= 0.0, x + lx, x)
+    y = y - np.trunc(y / ly) * ly
+    y = np.where(y <= 0.0, y + ly, y)
+    return x, y
+
+
 # ------------------------------------------------------------------ 3D MHD
 class MhdParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("lx", "ly", "lz", "mu", "eta", "cs", "dt")]

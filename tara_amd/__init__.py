@@ -157,6 +157,8 @@ def lib():
         L.tara_mhd_ic_abc.argtypes = [C.c_void_p] + [C.c_double] * 8
         L.tara_mhd_ic_random.argtypes = [C.c_void_p, C.c_uint] + [C.c_double] * 3
         L.tara_mhd_energy.argtypes = [C.c_void_p, _dp]
+        L.tara_csh_advance_tracers.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                               C.c_int, _dp, _dp]
         L.tara_mhd_record_energy.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         L.tara_derive2csh_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
@@ -272,7 +274,7 @@ class CshSolver:
     (alpha, lam as in derive2css)."""
 
     def __init__(self, nx, ny, nu, dt, scheme=SCHEME_AB2, screened=False, alpha=1.0, lam=CSS_LAMBDA):
-        self.nx, self.ny, self.nh = nx, ny, nx // 2 + 1
+        self.nx, self.ny, self.nh, self.dt = nx, ny, nx // 2 + 1, dt
         h = C.c_void_p()
         if screened:
             _check(lib().tara_css_create(nx, ny, alpha, nu, lam, dt, scheme, C.byref(h)))
@@ -328,6 +330,17 @@ class CshSolver:
         frames = np.zeros((nframes, self.ny, self.nh), np.complex128)
         _check(lib().tara_csh_record(self._h, nframes, every, frames.ctypes.data_as(_dp)))
         return [np.asfortranarray(f.T) for f in frames]
+
+    def advance_tracers(self, nsteps, xp, yp, lx=2 * np.pi, ly=2 * np.pi, dt=None, periodic=False):
+        """Steps nsteps times, moving the tracer particles (xp, yp: float64
+        arrays in [0, lx] x [0, ly], updated in place) after each step as
+        src/main2csh.f95:187-265; periodic=False reads the interpolation
+        corners as the driver addresses its arrays, True takes their periodic
+        images."""
+        assert xp.dtype == np.float64 and yp.dtype == np.float64 and xp.size == yp.size
+        assert xp.flags.c_contiguous and yp.flags.c_contiguous
+        _check(lib().tara_csh_advance_tracers(self._h, nsteps, self.dt if dt is None else dt, lx, ly,
+                                              int(bool(periodic)), xp.size, _p(xp), _p(yp)))
 
     def ic_shear(self, lx_in=2.0, ly_in=2.0, w0=25.0, m0=2.0, d_in=3.0 / 128.0):
         """The reference's shear layer generated on the device (history zeroed)."""

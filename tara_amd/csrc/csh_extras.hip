@@ -84,6 +84,82 @@ __global__ void k_random_phase(double2* w, int nh, int ny, unsigned seed, double
     }
 }
 
+// ------------------------------------------------------------ tracers
+// Velocity spectra of the driver (src/main2csh.f95:128-156): kx = 2 pi (i-1)/Lx,
+// ky = 2 pi (j-1)/Ly or 2 pi ((j-1)-Ny)/Ly with the caller's box and true pi,
+// psi = omega/(kx^2+ky^2) (psi(0,0) = 0), ux = +i ky psi, uy = -i kx psi, from
+// the ky-major state into two padded kx-fastest planes (pitch ld).
+__global__ void k_tracer_spec(const double2* w, int nh, int ny, int ld, double lx, double ly, double2* s)
+{
+    const double pi = 3.14159265358979323846;
+    const size_t plane = (size_t)ld * ny;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)nh * ny;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(q % ny), i = (int)(q / ny);
+        const double kx = 2.0 * pi * (double)i / lx;
+        const double ky = 2.0 * pi * (double)(j < ny / 2 ? j : j - ny) / ly;
+        const double2 z = w[q];
+        const double k2 = kx * kx + ky * ky;
+        const double2 ps = (i == 0 && j == 0) ? make_double2(0.0, 0.0) : make_double2(z.x / k2, z.y / k2);
+        const size_t o = (size_t)j * ld + i;
+        s[o] = make_double2(-(ky * ps.y), ky * ps.x);
+        s[plane + o] = make_double2(kx * ps.y, -(kx * ps.x));
+    }
+}
+
+// One step of the tracer particles (src/main2csh.f95:187-265): bilinear
+// interpolation of ux, uy (unnormalised c2r output, divided by Nx*Ny as the
+// driver does) at each particle with the driver's expression, then
+// x <- x + u dt and the wrap into (0, L].  The corners are read as the
+// driver addresses its column-major (Nx, Ny) arrays: with periodic == 0,
+// ux(m+1, n) of the last column is the element after ux(Nx, n) in memory,
+// ux(1, n+1), as there, and reads past the array's end (last row, where the
+// driver's are undefined) wrap to its start, ux(m, Ny+1) -> ux(m, 1); with
+// periodic != 0 every corner is the periodic image.  The last cell uses
+// x(Nx+1) = Lx, y(Ny+1) = Ly as the driver's edge branches do.  A position
+// that is not finite or lies far outside the box sets *bad and is left alone.
+__global__ void k_tracers(const double* ux, const double* uy, int nx, int ny, double lx, double ly, double dt,
+                          int periodic, int np, double* xp, double* yp, int* bad)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double dx = lx / (double)nx, dy = ly / (double)ny, h = 1.0 / (dx * dy), nn = (double)nx * (double)ny;
+    double x = xp[p], y = yp[p];
+    if (!(fabs(x) <= 4.0 * lx && fabs(y) <= 4.0 * ly)) {
+        *bad = 1;
+        return;
+    }
+    int m = (int)(x / dx), n = (int)(y / dy);  // 0-based cells: aint(x/dx) + 1 - 1
+    m = m < 0 ? 0 : (m >= nx ? nx - 1 : m);
+    n = n < 0 ? 0 : (n >= ny ? ny - 1 : n);
+    const double xm = (double)m * dx, ym = (double)n * dy;
+    const double xm1 = m + 1 < nx ? (double)(m + 1) * dx : lx, yn1 = n + 1 < ny ? (double)(n + 1) * dy : ly;
+    const size_t total = (size_t)nx * ny, a = (size_t)n * nx + m;
+    size_t b, c, d;
+    if (periodic) {
+        const size_t m1 = (size_t)((m + 1) % nx), n1 = (size_t)((n + 1) % ny);
+        b = (size_t)n * nx + m1;
+        c = n1 * nx + m;
+        d = n1 * nx + m1;
+    } else {
+        b = (a + 1) % total;
+        c = (a + nx) % total;
+        d = (a + nx + 1) % total;
+    }
+    const double vx = (ux[a] / nn) * (xm1 - x) * (yn1 - y) * h + (ux[b] / nn) * (x - xm) * (yn1 - y) * h +
+                      (ux[c] / nn) * (xm1 - x) * (y - ym) * h + (ux[d] / nn) * (x - xm) * (y - ym) * h;
+    const double vy = (uy[a] / nn) * (xm1 - x) * (yn1 - y) * h + (uy[b] / nn) * (x - xm) * (yn1 - y) * h +
+                      (uy[c] / nn) * (xm1 - x) * (y - ym) * h + (uy[d] / nn) * (x - xm) * (y - ym) * h;
+    x = x + vx * dt;
+    y = y + vy * dt;
+    x = x - trunc(x / lx) * lx;  // int(x/Lx) without the integer overflow
+    if (x <= 0.0) x = x + lx;
+    y = y - trunc(y / ly) * ly;
+    if (y <= 0.0) y = y + ly;
+    xp[p] = x;
+    yp[p] = y;
+}
+
 static unsigned grid_for(size_t n)
 {
     size_t b = (n + 255) / 256;
@@ -136,6 +212,58 @@ extern "C" int tara_csh_fields(tara_csh* c, double* omega, double* ux, double* u
     for (int f = 0; f < 3; ++f)
         if (outs[f]) TARA_HIP(hipMemcpyAsync(outs[f], real.as<double>() + f * nr, nr * 8, hipMemcpyDeviceToHost, st));
     TARA_HIP(hipStreamSynchronize(st));
+    return TARA_OK;
+}
+
+extern "C" int tara_csh_advance_tracers(tara_csh* c, int nsteps, double dt, double lx, double ly, int periodic,
+                                        int np, double* xp, double* yp)
+{
+    TARA_CHECK(c && nsteps >= 0 && np >= 0 && (np == 0 || (xp && yp)) && lx > 0 && ly > 0 && std::isfinite(dt),
+               TARA_ERR_ARG, "bad argument");
+    for (int i = 0; i < np; ++i)
+        TARA_CHECK(xp[i] >= 0.0 && xp[i] <= lx && yp[i] >= 0.0 && yp[i] <= ly, TARA_ERR_ARG,
+                   "tracers: a position lies outside [0, lx] x [0, ly]");
+    double2* w = nullptr;
+    const CshPlan* p = nullptr;
+    hipStream_t st = nullptr;
+    int rc = csh_ctx_state(c, &w, &p, &st);
+    if (rc) return rc;
+    const int nx = p->nx, ny = p->ny, ld = spec_ld(nx);
+    TARA_CHECK(nx >= 16 && ny >= 8, TARA_ERR_UNSUPPORTED, "tracers: grid too small");
+    DevBuf spec, real, pos, flag;
+    if ((rc = spec.alloc(2 * (size_t)ld * ny * sizeof(double2))) || (rc = real.alloc(2 * (size_t)nx * ny * 8)) ||
+        (rc = pos.alloc(2 * (size_t)(np ? np : 1) * 8)) || (rc = flag.alloc(sizeof(int))))
+        return rc;
+    double* dxp = pos.as<double>();
+    double* dyp = dxp + np;
+    if (np) {
+        TARA_HIP(hipMemcpyAsync(dxp, xp, (size_t)np * 8, hipMemcpyHostToDevice, st));
+        TARA_HIP(hipMemcpyAsync(dyp, yp, (size_t)np * 8, hipMemcpyHostToDevice, st));
+    }
+    TARA_HIP(hipMemsetAsync(spec.p, 0, spec.bytes, st));
+    TARA_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+    const size_t nr = (size_t)nx * ny;
+    for (int k = 0; k < nsteps; ++k) {
+        if ((rc = tara_csh_advance(c, 1))) return rc;
+        hipLaunchKernelGGL(k_tracer_spec, dim3(grid_for((size_t)p->nh * ny)), dim3(256), 0, st, w, p->nh, ny, ld, lx,
+                           ly, spec.as<double2>());
+        TARA_HIP(hipGetLastError());
+        if ((rc = col_fft(spec.as<double2>(), ny, ld, ld, (long)ld * ny, 2, +1, st))) return rc;
+        if ((rc = row_c2r(spec.as<double2>(), real.as<double>(), nx, ld, 2 * ny, 1.0, st))) return rc;
+        if (np) {
+            hipLaunchKernelGGL(k_tracers, dim3((np + 255) / 256), dim3(256), 0, st, real.as<double>(),
+                               real.as<double>() + nr, nx, ny, lx, ly, dt, periodic, np, dxp, dyp, flag.as<int>());
+            TARA_HIP(hipGetLastError());
+        }
+    }
+    int bad = 0;
+    if (np) {
+        TARA_HIP(hipMemcpyAsync(xp, dxp, (size_t)np * 8, hipMemcpyDeviceToHost, st));
+        TARA_HIP(hipMemcpyAsync(yp, dyp, (size_t)np * 8, hipMemcpyDeviceToHost, st));
+    }
+    TARA_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    TARA_HIP(hipStreamSynchronize(st));
+    TARA_CHECK(!bad, TARA_ERR_NONFINITE, "tracers: a particle position became non-finite");
     return TARA_OK;
 }
 

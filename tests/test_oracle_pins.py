@@ -228,3 +228,95 @@ def test_ic_random3d_properties():
     assert np.max(ratio) / np.min(ratio) < 2.0
     _, _, slope3 = _shell_fit(k, e, range(2, 9), lambda s: s ** 3 * np.exp(-2.0 * (s / 3.0) ** 2))
     assert slope3 > 0.7
+
+
+# ------------------------------------------------------------ tracer particles
+@pytest.mark.parametrize("lx_in,ly_in,p,q", [(2.0, 2.0, 2, 3), (4.0, 2.0, 3, -2), (2.0, 4.0, 0, 5), (2.0, 2.0, 4, 0)])
+def test_csh_velocity_single_mode(lx_in, ly_in, p, q):
+    """omega = A cos(kx x + ky y) with kx = 2 pi p/Lx, ky = 2 pi q/Ly: psi =
+    omega/k^2, ux = d psi/dy = -A ky sin/k^2, uy = -d psi/dx = A kx sin/k^2
+    (the driver's i ky psi, -i kx psi); a swapped axis, a lost sign or a
+    wrong box scale fails one of the four cases."""
+    nx, ny, a = 32, 16, 1.7
+    lx, ly = lx_in * PI, ly_in * PI
+    kx, ky = 2 * PI * p / lx, 2 * PI * q / ly
+    x = (np.arange(nx) * lx / nx)[:, None]
+    y = (np.arange(ny) * ly / ny)[None, :]
+    th = kx * x + ky * y
+    w = O.r2c(a * np.cos(th))
+    ux, uy = O.csh_velocity(w, nx, lx, ly)
+    k2 = kx * kx + ky * ky
+    assert np.max(np.abs(ux - (-a * ky * np.sin(th) / k2))) < 1e-13
+    assert np.max(np.abs(uy - (a * kx * np.sin(th) / k2))) < 1e-13
+
+
+def _particles(rng, n, lx, ly):
+    return rng.uniform(0, lx, n), rng.uniform(0, ly, n)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_tracer_step_linear_field_is_exact(periodic):
+    """Bilinear interpolation reproduces a field linear in the grid indices,
+    f(i, j) = alpha i + beta j + gamma, exactly inside the grid (cells below
+    the last column and row): v = alpha x/dx + beta y/dy + gamma."""
+    nx, ny, lx, ly, dt = 16, 8, 2 * PI, 4 * PI, 1e-3
+    dx, dy = lx / nx, ly / ny
+    i = np.arange(nx)[:, None]
+    j = np.arange(ny)[None, :]
+    ux = np.asfortranarray(0.3 * i - 0.7 * j + 0.2)
+    uy = np.asfortranarray(-0.4 * i + 0.9 * j - 0.1)
+    rng = np.random.default_rng(3)
+    xp = rng.uniform(0, (nx - 1) * dx, 200)
+    yp = rng.uniform(0, (ny - 1) * dy, 200)
+    x, y = O.tracer_step(ux, uy, xp, yp, lx, ly, dt, periodic)
+    vx = 0.3 * xp / dx - 0.7 * yp / dy + 0.2
+    vy = -0.4 * xp / dx + 0.9 * yp / dy - 0.1
+    assert np.max(np.abs(x - (xp + vx * dt))) < 1e-13
+    assert np.max(np.abs(y - (yp + vy * dt))) < 1e-13
+
+
+def _pdist(a, b, period):
+    d = np.mod(a - b + 0.5 * period, period) - 0.5 * period
+    return np.abs(d)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_tracer_step_uniform_flow_and_wrap(periodic):
+    """A uniform flow moves every particle by (U dt, V dt) whatever its cell
+    (the weights sum to one, the last cell's included), and the wrap keeps
+    positions in (0, L]: particles that cross either edge re-enter."""
+    nx, ny, lx, ly = 16, 16, 2 * PI, 2 * PI
+    U, V, dt = 3.0, -2.0, 0.05
+    ux = np.full((nx, ny), U, order="F")
+    uy = np.full((nx, ny), V, order="F")
+    rng = np.random.default_rng(4)
+    xp, yp = _particles(rng, 300, lx, ly)
+    xp[:20] = lx - rng.uniform(0, 0.1, 20)   # crosses x = Lx
+    yp[20:40] = rng.uniform(0, 0.05, 20)     # crosses y = 0
+    xp[40], yp[40] = lx, ly                  # the corner itself
+    x, y = O.tracer_step(ux, uy, xp, yp, lx, ly, dt, periodic)
+    assert np.all((x > 0) & (x <= lx) & (y > 0) & (y <= ly))
+    assert np.max(_pdist(x, xp + U * dt, lx)) < 1e-13
+    assert np.max(_pdist(y, yp + V * dt, ly)) < 1e-13
+    assert np.sum(x < xp) >= 20 and np.sum(y > yp) >= 20
+
+
+def test_tracer_step_last_column_addressing():
+    """In the last column the driver's right-hand corner ux(Nx+1, n) is, in
+    column-major memory, ux(1, n+1) (next row); the periodic form uses
+    ux(1, n).  For a field that depends on j only, f(i, j) = j, a particle at
+    x in the last column and y = n dy + s dy gets n + s (periodic) and
+    n + s + f_x (driver), f_x = (x - x(Nx))/dx its fraction across the cell;
+    inside the grid the two agree."""
+    nx, ny, lx, ly, dt = 16, 16, 2 * PI, 2 * PI, 1e-3
+    dx, dy = lx / nx, ly / ny
+    f = np.asfortranarray(np.tile(np.arange(ny, dtype=float), (nx, 1)))
+    z = np.zeros((nx, ny), order="F")
+    n, fx, s = 5, 0.25, 0.5
+    xp = np.array([(nx - 1 + fx) * dx, 3.5 * dx])
+    yp = np.array([(n + s) * dy, (n + s) * dy])
+    _, yd = O.tracer_step(z, f, xp, yp, lx, ly, dt, periodic=False)
+    _, yq = O.tracer_step(z, f, xp, yp, lx, ly, dt, periodic=True)
+    assert abs((yq[0] - yp[0]) / dt - (n + s)) < 1e-9
+    assert abs((yd[0] - yp[0]) / dt - (n + s + fx)) < 1e-9
+    assert yd[1] == yq[1] and abs((yq[1] - yp[1]) / dt - (n + s)) < 1e-9
