@@ -129,8 +129,8 @@ int tara_csh_record(tara_csh* ctx, int nframes, int every, double* frames);
  * values times pi); dt: the driver's step.  periodic == 0 reads the corners
  * as the driver addresses its column-major arrays (the last column's right
  * neighbour is the next row's first element; reads past the end, undefined
- * there, wrap to the start); periodic != 0 takes every corner's periodic
- * image.  Synchronous; TARA_ERR_NONFINITE if a position became non-finite. */
+ * there, read 0, as the driver's do when glibc allocates its arrays with
+ * mmap, from 128 KiB); periodic != 0 takes every corner's periodic image.  Synchronous; TARA_ERR_NONFINITE if a position became non-finite. */
 int tara_csh_advance_tracers(tara_csh* ctx, int nsteps, double dt, double lx, double ly, int periodic, int np,
                              double* xp, double* yp);
 /* Initial conditions generated on the device into the context's state (the AB2

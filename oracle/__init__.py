@@ -197,9 +197,10 @@ def tracer_step(ux, uy, xp, yp, lx, ly, dt, periodic=False):
     the last column, y(n+1) = Ly in the last row), x <- x + v dt, then
     x <- x - int(x/Lx) Lx and + Lx if <= 0.  Corners: with periodic=False as
     the driver addresses its column-major arrays (flat index m + n Nx, +1 for
-    the right neighbour, +Nx for the upper one), reads past the end wrapping
-    to the start (the driver's are undefined there); with periodic=True the
-    periodic images.  A cell index at or past Nx (xp = Lx) is taken as the
+    the right neighbour, +Nx for the upper one), a read past the end giving 0
+    (undefined in the language; what the driver reads from the zero-filled
+    rest of the last page when its arrays are mmap-allocated); with
+    periodic=True the periodic images.  A cell index at or past Nx (xp = Lx) is taken as the
     last cell."""
     ux = np.asfortranarray(ux)
     uy = np.asfortranarray(uy)
@@ -213,14 +214,14 @@ def tracer_step(ux, uy, xp, yp, lx, ly, dt, periodic=False):
     xm, ym = m * dx, n * dy
     xm1 = np.where(m + 1 < nx, (m + 1) * dx, lx)
     yn1 = np.where(n + 1 < ny, (n + 1) * dy, ly)
-    total = nx * ny
     a = m + n * nx
     if periodic:
         m1, n1 = (m + 1) % nx, (n + 1) % ny
         b, c, d = m1 + n * nx, m + n1 * nx, m1 + n1 * nx
     else:
-        b, c, d = (a + 1) % total, (a + nx) % total, (a + nx + 1) % total
-    fx, fy = ux.ravel(order="F"), uy.ravel(order="F")
+        b, c, d = a + 1, a + nx, a + nx + 1
+    fx = np.concatenate([ux.ravel(order="F"), np.zeros(nx + 1)])
+    fy = np.concatenate([uy.ravel(order="F"), np.zeros(nx + 1)])
 
     def interp(f):
         return (f[a] * (xm1 - xp) * (yn1 - yp) * h + f[b] * (xp - xm) * (yn1 - yp) * h

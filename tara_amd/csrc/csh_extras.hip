@@ -113,11 +113,14 @@ __global__ void k_tracer_spec(const double2* w, int nh, int ny, int ld, double l
 // x <- x + u dt and the wrap into (0, L].  The corners are read as the
 // driver addresses its column-major (Nx, Ny) arrays: with periodic == 0,
 // ux(m+1, n) of the last column is the element after ux(Nx, n) in memory,
-// ux(1, n+1), as there, and reads past the array's end (last row, where the
-// driver's are undefined) wrap to its start, ux(m, Ny+1) -> ux(m, 1); with
-// periodic != 0 every corner is the periodic image.  The last cell uses
-// x(Nx+1) = Lx, y(Ny+1) = Ly as the driver's edge branches do.  A position
-// that is not finite or lies far outside the box sets *bad and is left alone.
+// ux(1, n+1), as there, and a read past the array's end (the last row's
+// upper corners, the last column's in the row below it) reads 0: undefined
+// in the language, it is what the driver reads where its arrays are
+// allocated with mmap, as glibc does for arrays of 128 KiB and more (the
+// page holding the end is zero-filled beyond it).  With periodic != 0 every
+// corner is the periodic image.  The last cell uses x(Nx+1) = Lx,
+// y(Ny+1) = Ly as the driver's edge branches do.  A position that is not
+// finite or lies far outside the box sets *bad and is left alone.
 __global__ void k_tracers(const double* ux, const double* uy, int nx, int ny, double lx, double ly, double dt,
                           int periodic, int np, double* xp, double* yp, int* bad)
 {
@@ -142,14 +145,18 @@ __global__ void k_tracers(const double* ux, const double* uy, int nx, int ny, do
         c = n1 * nx + m;
         d = n1 * nx + m1;
     } else {
-        b = (a + 1) % total;
-        c = (a + nx) % total;
-        d = (a + nx + 1) % total;
+        b = a + 1;
+        c = a + nx;
+        d = a + nx + 1;
     }
-    const double vx = (ux[a] / nn) * (xm1 - x) * (yn1 - y) * h + (ux[b] / nn) * (x - xm) * (yn1 - y) * h +
-                      (ux[c] / nn) * (xm1 - x) * (y - ym) * h + (ux[d] / nn) * (x - xm) * (y - ym) * h;
-    const double vy = (uy[a] / nn) * (xm1 - x) * (yn1 - y) * h + (uy[b] / nn) * (x - xm) * (yn1 - y) * h +
-                      (uy[c] / nn) * (xm1 - x) * (y - ym) * h + (uy[d] / nn) * (x - xm) * (y - ym) * h;
+    const double uxa = ux[a], uya = uy[a];
+    const double uxb = b < total ? ux[b] : 0.0, uyb = b < total ? uy[b] : 0.0;
+    const double uxc = c < total ? ux[c] : 0.0, uyc = c < total ? uy[c] : 0.0;
+    const double uxd = d < total ? ux[d] : 0.0, uyd = d < total ? uy[d] : 0.0;
+    const double vx = (uxa / nn) * (xm1 - x) * (yn1 - y) * h + (uxb / nn) * (x - xm) * (yn1 - y) * h +
+                      (uxc / nn) * (xm1 - x) * (y - ym) * h + (uxd / nn) * (x - xm) * (y - ym) * h;
+    const double vy = (uya / nn) * (xm1 - x) * (yn1 - y) * h + (uyb / nn) * (x - xm) * (yn1 - y) * h +
+                      (uyc / nn) * (xm1 - x) * (y - ym) * h + (uyd / nn) * (x - xm) * (y - ym) * h;
     x = x + vx * dt;
     y = y + vy * dt;
     x = x - trunc(x / lx) * lx;  // int(x/Lx) without the integer overflow

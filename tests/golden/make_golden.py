@@ -7,7 +7,7 @@ are produced by running its routines on the inputs below.  Run here (where
 /root/reference exists); the .npz files travel with the repository, the
 reference does not.
 
-    ulimit -s unlimited && python tests/golden/make_golden.py [core|fam2d|long|all]
+    ulimit -s unlimited && python tests/golden/make_golden.py [core|fam2d|long|snap|all]
 
 (derive3com keeps ~130 automatic arrays on the stack, hence the stack limit.)
 """
@@ -245,8 +245,140 @@ def main_long():
     print("wrote", len(out), "arrays to golden_long.npz")
 
 
+SNAP_INI = """[dimension]
+dim = 2
+[architecture]
+arch = cpu
+[parallel]
+mode = serial
+[type]
+domain = hydro
+[problem]
+hydrotype = bare
+[particle]
+Np = 256
+[grid]
+Nx = 64
+Ny = 64
+Nz = 64
+[length]
+Lx = 2.0d0
+Ly = 2.0d0
+Lz = 2.0d0
+[resistivity]
+nu = 0.010d0
+eta = 0.0d0
+[time]
+time_min = 0.0d0
+time_max = 1.0005d0
+dt = 0.0010d0
+[initialProfile]
+W0 = 25.0d0
+m0 = 2.0d0
+d = 0.0234375d0
+[initialB]
+B0 = 0.0d0
+"""
+
+
+def read_fort(path, ncol):
+    """A list-directed text file (records may wrap) as an (n, ncol) array."""
+    with open(path) as f:
+        return np.array(f.read().split(), dtype=float).reshape(-1, ncol)
+
+
+def snap_ini(n, np_, trips):
+    """input.ini of the config-1 program (dispatcher -> tara2dHydSer): n^2,
+    np_ particles, `trips` loop trips (time_max between two multiples of dt)."""
+    return (SNAP_INI.replace("Nx = 64", f"Nx = {n}").replace("Ny = 64", f"Ny = {n}")
+            .replace("Np = 256", f"Np = {np_}").replace("time_max = 1.0005d0", f"time_max = {(trips - 0.5) * 1e-3!r}d0"))
+
+
+def run_snap(exe, ini, times):
+    """Runs a build of the config-1 program on `ini` in a scratch directory;
+    returns {t: (fields, particles)} of its text snapshots at the given t."""
+    import shutil
+    import tempfile
+    work = tempfile.mkdtemp()
+    try:
+        os.mkdir(os.path.join(work, "output"))
+        with open(os.path.join(work, "input.ini"), "w") as f:
+            f.write(ini)
+        subprocess.run([exe, "input.ini"], cwd=work, check=True, capture_output=True, timeout=600)
+        return {t: (read_fort(os.path.join(work, "output", f"fort.{t + 100:08d}"), 5),
+                    read_fort(os.path.join(work, "output", f"fort.{t + 105:08d}"), 2)) for t in times}
+    finally:
+        shutil.rmtree(work)
+
+
+def rand_starts(np_, lx):
+    """main2csh's start positions (src/main2csh.f95:93-96): xp = RAND() Lx,
+    yp = RAND() Ly in turn, from RAND of libtara_gnucompat, the generator the
+    linked program calls, at its initial state."""
+    import ctypes
+    root = os.path.dirname(os.path.dirname(HERE))
+    g = ctypes.CDLL(os.path.join(root, "tara_amd", "libtara_gnucompat.so"))
+    g.rand_.restype = ctypes.c_float
+    g.srand_(ctypes.byref(ctypes.c_int(1)))
+    start = np.zeros((np_, 2))
+    for i in range(np_):
+        start[i, 0] = float(g.rand_()) * lx
+        start[i, 1] = float(g.rand_()) * lx
+    return start
+
+
+def main_snap():
+    """The reference's config-1 program itself (src/main.f95 -> tara2dHydSer,
+    src/main2csh.f95), built against the reference routines and FFTW 3.3.9
+    (tests/fortran/_mains/tara_snap_ref; its snapshot OPENs made flang-safe,
+    see tests/fortran/Makefile.mains), and its text snapshots (fields: x, y,
+    omega, ux, uy per point; particles: x, y):
+      s64_*  : 64^2 shear layer, 256 particles, 1001 loop trips, snapshots at
+               t = 0 and t = 1000;
+      s128_* : 128^2, 2048 particles, one trip, the particles only.  At this
+               size the program's arrays are 128 KiB, which glibc maps with
+               mmap, so the tracer code's reads past their end (last row) read
+               the zero-filled rest of the last page; at 64^2 they read
+               whatever the heap holds next (the following array).
+    The AB2 history the program leaves uninitialised is zero in both (memory
+    fresh from the kernel)."""
+    root = os.path.dirname(os.path.dirname(HERE))
+    exe = os.path.join(root, "tests", "fortran", "_mains", "tara_snap_ref")
+    subprocess.run(["make", "-s", "-f", os.path.join(root, "tests", "fortran", "Makefile.mains")], check=True)
+    lx = 2.0 * PI
+    out = {}
+    ini = snap_ini(64, 256, 1001)
+    r = run_snap(exe, ini, (0, 1000))
+    out["s64_ini"] = np.array(ini)
+    out["s64_start"] = rand_starts(256, lx)
+    for t in (0, 1000):
+        out[f"s64_fields_{t}"], out[f"s64_particles_{t}"] = r[t]
+    # which particles enter the last row during the run (there the program
+    # reads the heap past its arrays): the oracle's trajectory, which equals
+    # the program's for the others
+    _, w = O.ic2csh(64, 64, d_in=0.0234375)
+    hist = np.zeros_like(w, order="F")
+    xp, yp = out["s64_start"][:, 0].copy(), out["s64_start"][:, 1].copy()
+    last_row = np.zeros(256, bool)
+    for _ in range(1001):
+        w, hist = O.ab2csh_steps(w, hist, 64, 0.01, 1e-3, 1)
+        ux, uy = O.csh_velocity(w, 64, lx, lx)
+        last_row |= np.trunc(yp / (lx / 64)) >= 63
+        xp, yp = O.tracer_step(ux, uy, xp, yp, lx, lx, 1e-3)
+    out["s64_last_row"] = last_row
+    ini = snap_ini(128, 2048, 1)
+    r = run_snap(exe, ini, (0,))
+    out["s128_ini"] = np.array(ini)
+    out["s128_start"] = rand_starts(2048, lx)
+    out["s128_particles_0"] = r[0][1]
+    np.savez_compressed(os.path.join(HERE, "golden_snap.npz"), **out)
+    print("wrote", len(out), "arrays to golden_snap.npz")
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("snap", "all"):
+        main_snap()
     if what in ("long", "all"):
         main_long()
     if what in ("core", "all"):

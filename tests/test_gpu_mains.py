@@ -9,15 +9,21 @@ oracle/Makefile).  Both write output/Energy.dat once per step (pressure,
 kinetic and total energy, B^2, vorticity moments, helicities and div B,
 src/main3com.f95:1024-1071); the runs agree to the FP64 parity bar.
 
-Both binaries are built by __graft_entry__.build() where the reference
-checkout is present.  The reference allocates its AB2 history uninitialised
-(src/main3com.f95 allocates d_*_dt_old without setting it); both runs set
-glibc's mmap threshold to one page so that allocations come from fresh,
-zeroed pages, the zero history both assume.
+The config-1 program (dispatcher -> tara2dHydSer, src/main2csh.f95) runs
+the same way: tests/fortran/_mains/tara_snap, the main compiled with its two
+snapshot OPEN statements made flang-safe (flang's runtime stops on their
+re-opening of a connected unit with STATUS='unknown', src/main2csh.f95:177,
+:268; gfortran tolerates it), derive2csh / ab2csh through the wrappers and its
+three c2r per step through the dfftw shim; its text snapshots (fields and
+tracer particles) are compared with those of the program built against the
+reference routines (tests/golden/golden_snap.npz).
 
-(The 2D mains link too, tests/test_link_mains.py, but flang's runtime stops
-them at their first text snapshot: they re-open an already connected unit with
-STATUS='unknown', e.g. src/main2csh.f95:177, which gfortran tolerates.)"""
+All binaries are built by __graft_entry__.build() where the reference checkout
+is present.  The reference allocates its AB2 history uninitialised
+(src/main3com.f95 allocates d_*_dt_old, src/main2csh.f95 dt_omegak_old
+without setting them); at these sizes that memory is fresh from the kernel,
+hence zero, the zero history both paths assume (MALLOC_MMAP_THRESHOLD_ is
+also set, where glibc honours it)."""
 import os
 import subprocess
 
@@ -28,6 +34,7 @@ from conftest import ROOT
 
 MAINS = os.path.join(ROOT, "tests", "fortran", "_mains")
 TARA, TARA_REF = os.path.join(MAINS, "tara"), os.path.join(MAINS, "tara_ref")
+TARA_SNAP = os.path.join(MAINS, "tara_snap")
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not (os.path.exists(TARA) and os.path.exists(TARA_REF)),
                                  reason="reference mains not linked (no reference checkout at build)")]
@@ -126,3 +133,40 @@ def test_device_diagnostics_vs_reference_program(tmp_path, n, steps):
         assert err < 1e-10, (name, err)
     bscale = np.max(np.abs(want[:, 4]))
     assert np.max(np.abs(got[:, -1])) < 1e-12 * bscale
+
+
+def _read_fort(path, ncol):
+    with open(path) as f:
+        return np.array(f.read().split(), dtype=float).reshape(-1, ncol)
+
+
+@pytest.mark.skipif(not os.path.exists(TARA_SNAP), reason="config-1 program not linked")
+def test_reference_main2csh_through_tara_vs_reference_program(tmp_path):
+    """The config-1 program linked against tara_amd (GPU) on a 64^2 shear
+    layer with 256 tracer particles, 1001 loop trips, against the same
+    program built against the reference routines and FFTW: its snapshots of
+    x, y, omega, ux, uy at t = 0 and t = 1000 to 1e-10 of each field's scale,
+    and the particle positions (those that never enter the last row, where
+    the program reads past its arrays into the heap) to 1e-10."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "golden_snap.npz"))
+    (tmp_path / "output").mkdir()
+    (tmp_path / "input.ini").write_text(str(g["s64_ini"]))
+    print("main2csh 64^2 x 1001 linked against tara_amd (GPU)", flush=True)
+    env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="4096")
+    r = subprocess.run([TARA_SNAP, "input.ini"], cwd=tmp_path, env=env, capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    ok = ~g["s64_last_row"]
+    for t in (0, 1000):
+        f = _read_fort(tmp_path / "output" / f"fort.{t + 100:08d}", 5)
+        want = g[f"s64_fields_{t}"]
+        assert np.array_equal(f[:, :2], want[:, :2])
+        for c in (2, 3, 4):
+            err = np.max(np.abs(f[:, c] - want[:, c])) / np.max(np.abs(want[:, c]))
+            print(f"  t={t} column {c}: max rel diff {err:.2e}", flush=True)
+            assert err < 1e-10, (t, c, err)
+        p = _read_fort(tmp_path / "output" / f"fort.{t + 105:08d}", 2)
+        want = g[f"s64_particles_{t}"]
+        err = np.max(np.abs(p[ok] - want[ok]))
+        print(f"  t={t} particles: max diff {err:.2e}", flush=True)
+        assert err < 1e-10, (t, err)

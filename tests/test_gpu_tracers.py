@@ -1,12 +1,16 @@
 """Device tracer particles (src/main2csh.f95:187-265) through the C ABI:
-against the oracle's restatement step by step over a run of the driver's
-shear-layer flow, and against the closed form of a steady shear flow."""
+against the reference program's own text snapshots (fixture written by
+tests/golden/make_golden.py snap), against the oracle's restatement step by
+step over a run of the driver's shear-layer flow, and against the closed form
+of a steady shear flow."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
 import tara_amd as T
-from conftest import rel_l2
+from conftest import ROOT, rel_l2
 
 pytestmark = pytest.mark.gpu
 L = 2.0 * np.pi
@@ -28,6 +32,54 @@ def _particles(nx, ny, lx, ly, n, seed):
     yp[32:36] = ly - dy * rng.uniform(0.01, 0.99, 4)
     xp[36], xp[37] = 0.0, lx
     return np.ascontiguousarray(xp), np.ascontiguousarray(yp)
+
+
+def _golden_snap():
+    return np.load(os.path.join(ROOT, "tests", "golden", "golden_snap.npz"))
+
+
+def test_tracers_and_fields_vs_reference_program():
+    """The reference program's config-1 run (64^2 shear layer, nu = 0.01,
+    dt = 1e-3, AB2, 256 particles) against the device: omega, ux, uy and the
+    positions after 1001 steps (one advance_tracers call), and after 1 step.
+    Particles that enter the last row are left out (the program reads past
+    its 32 KiB arrays into the heap there; the fixture marks them)."""
+    g = _golden_snap()
+    nx, nu, dt = 64, 0.01, 1e-3
+    print("tracers vs reference program 64^2", flush=True)
+    for steps, t in ((1, 0), (1001, 1000)):
+        xd = np.ascontiguousarray(g["s64_start"][:, 0])
+        yd = np.ascontiguousarray(g["s64_start"][:, 1])
+        s = T.CshSolver(nx, nx, nu, dt, T.SCHEME_AB2)
+        s.ic_shear(2.0, 2.0, 25.0, 2.0, 0.0234375)
+        s.advance_tracers(steps, xd, yd, L, L)
+        om, ux, uy = s.fields()
+        s.close()
+        f = g[f"s64_fields_{t}"]
+        for col, got in ((2, om), (3, ux), (4, uy)):
+            ref = f[:, col].reshape(nx, nx)
+            assert np.max(np.abs(got - ref)) < 1e-10 * np.max(np.abs(ref)), (t, col)
+        p = g[f"s64_particles_{t}"]
+        ok = ~g["s64_last_row"]
+        err = max(np.max(np.abs(p[ok, 0] - xd[ok])), np.max(np.abs(p[ok, 1] - yd[ok])))
+        assert err < 1e-10, (t, err)
+
+
+def test_tracers_past_the_end_reads_vs_reference_program():
+    """One step of 2048 particles at 128^2 against the reference program,
+    every particle (last row, last column and corner cell included: the
+    program's arrays are mmap-allocated there, so its reads past their end
+    give 0, as the device's driver addressing)."""
+    g = _golden_snap()
+    nx = 128
+    xd = np.ascontiguousarray(g["s128_start"][:, 0])
+    yd = np.ascontiguousarray(g["s128_start"][:, 1])
+    s = T.CshSolver(nx, nx, 0.01, 1e-3, T.SCHEME_AB2)
+    s.ic_shear(2.0, 2.0, 25.0, 2.0, 0.0234375)
+    s.advance_tracers(1, xd, yd, L, L)
+    s.close()
+    p = g["s128_particles_0"]
+    assert np.max(np.abs(p[:, 0] - xd)) < 1e-12 and np.max(np.abs(p[:, 1] - yd)) < 1e-12
 
 
 @pytest.mark.parametrize("periodic", [False, True])

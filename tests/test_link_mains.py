@@ -51,3 +51,7 @@ def test_reference_mains_link_against_tara_only(tmp_path):
     assert "fftw" not in ldd.replace("tara_dfftw", "")
     objs = os.listdir(out)
     assert not any(o.startswith(("derive", "ab2", "ab3", "rk4", "euler", "pc")) for o in objs), objs
+    # the config-1 program with flang-safe snapshot OPENs links the same way
+    snap = str(out / "tara_snap")
+    assert _symbols(snap, undefined=True) >= set(HOT_PATH[:2] + ["rand_"])
+    assert "fftw" not in subprocess.run(["ldd", snap], capture_output=True, text=True).stdout.replace("tara_dfftw", "")

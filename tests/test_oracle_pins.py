@@ -283,8 +283,10 @@ def _pdist(a, b, period):
 @pytest.mark.parametrize("periodic", [False, True])
 def test_tracer_step_uniform_flow_and_wrap(periodic):
     """A uniform flow moves every particle by (U dt, V dt) whatever its cell
-    (the weights sum to one, the last cell's included), and the wrap keeps
-    positions in (0, L]: particles that cross either edge re-enter."""
+    (the weights sum to one, the last cell's included; in the driver's
+    addressing, cells whose upper corners lie past the arrays' end excepted),
+    and the wrap keeps positions in (0, L]: particles that cross either edge
+    re-enter."""
     nx, ny, lx, ly = 16, 16, 2 * PI, 2 * PI
     U, V, dt = 3.0, -2.0, 0.05
     ux = np.full((nx, ny), U, order="F")
@@ -295,9 +297,10 @@ def test_tracer_step_uniform_flow_and_wrap(periodic):
     yp[20:40] = rng.uniform(0, 0.05, 20)     # crosses y = 0
     xp[40], yp[40] = lx, ly                  # the corner itself
     x, y = O.tracer_step(ux, uy, xp, yp, lx, ly, dt, periodic)
+    inside = np.ones(xp.size, bool) if periodic else yp < ly - 2 * (ly / ny)
     assert np.all((x > 0) & (x <= lx) & (y > 0) & (y <= ly))
-    assert np.max(_pdist(x, xp + U * dt, lx)) < 1e-13
-    assert np.max(_pdist(y, yp + V * dt, ly)) < 1e-13
+    assert np.max(_pdist(x, xp + U * dt, lx)[inside]) < 1e-13
+    assert np.max(_pdist(y, yp + V * dt, ly)[inside]) < 1e-13
     assert np.sum(x < xp) >= 20 and np.sum(y > yp) >= 20
 
 
@@ -320,3 +323,83 @@ def test_tracer_step_last_column_addressing():
     assert abs((yq[0] - yp[0]) / dt - (n + s)) < 1e-9
     assert abs((yd[0] - yp[0]) / dt - (n + s + fx)) < 1e-9
     assert yd[1] == yq[1] and abs((yq[1] - yp[1]) / dt - (n + s)) < 1e-9
+
+
+def test_tracer_step_past_the_end_reads_zero():
+    """In the driver's addressing the last row's upper corners, ux(m, Ny+1),
+    lie past the arrays' end and read 0: in a uniform field f = 1 a particle
+    at y = (Ny - 1 + s) dy (column m < Nx) moves with 1 - s, the weight of its
+    lower corners; the periodic form moves it with 1."""
+    nx, ny, lx, ly, dt = 16, 16, 2 * PI, 2 * PI, 1e-3
+    dx, dy = lx / nx, ly / ny
+    one = np.ones((nx, ny), order="F")
+    z = np.zeros((nx, ny), order="F")
+    s = 0.3
+    xp, yp = np.array([4.5 * dx]), np.array([(ny - 1 + s) * dy])
+    _, yd = O.tracer_step(z, one, xp, yp, lx, ly, dt, periodic=False)
+    _, yq = O.tracer_step(z, one, xp, yp, lx, ly, dt, periodic=True)
+    assert abs((yd[0] - yp[0]) / dt - (1 - s)) < 1e-9
+    assert abs((yq[0] - yp[0]) / dt - 1.0) < 1e-9
+
+
+def _golden_snap():
+    return np.load(os.path.join(ROOT, "tests", "golden", "golden_snap.npz"))
+
+
+def test_tracers_and_fields_vs_reference_program():
+    """The oracle's driver restatement (shear-layer IC, AB2 steps, velocity,
+    tracer step in the driver's addressing) against the reference program's
+    own text snapshots (src/main2csh.f95:170-178, :266-272; fixture from
+    tests/golden/make_golden.py snap): 64^2, 256 particles, omega, ux, uy and
+    the positions after 1 and 1001 steps.  Particles that never enter the
+    last row must match; there the program reads past its 32 KiB arrays into
+    whatever the heap holds next, which no restatement can know (the 128^2
+    case below pins those reads).  The compared ones visit the last column
+    (the aliased reads) many times."""
+    g = _golden_snap()
+    nx, lx, nu, dt = 64, 2 * PI, 0.01, 1e-3
+    _, w = O.ic2csh(nx, nx, d_in=0.0234375)
+    hist = np.zeros_like(w, order="F")
+    xp, yp = g["s64_start"][:, 0].copy(), g["s64_start"][:, 1].copy()
+    last_row = np.zeros(xp.size, bool)
+    last_col = 0
+    dx = lx / nx
+    for k in range(1001):
+        w, hist = O.ab2csh_steps(w, hist, nx, nu, dt, 1)
+        ux, uy = O.csh_velocity(w, nx, lx, lx)
+        last_row |= np.trunc(yp / dx) >= nx - 1
+        last_col += int(np.sum((np.trunc(xp / dx) >= nx - 1) & ~last_row))
+        xp, yp = O.tracer_step(ux, uy, xp, yp, lx, lx, dt, periodic=False)
+        if k in (0, 1000):
+            f = g[f"s64_fields_{k}"]
+            om = O.c2r(w, nx) / (nx * nx)
+            for col, ref in ((2, om), (3, ux), (4, uy)):
+                assert np.max(np.abs(f[:, col] - ref.ravel())) < 1e-12 * np.max(np.abs(ref)), (k, col)
+            p = g[f"s64_particles_{k}"]
+            ok = ~last_row
+            assert np.max(np.abs(p[ok, 0] - xp[ok])) < 1e-12 and np.max(np.abs(p[ok, 1] - yp[ok])) < 1e-12, k
+    assert np.sum(~last_row) >= 240 and last_col >= 100, (np.sum(~last_row), last_col)
+    assert np.array_equal(last_row, g["s64_last_row"])
+
+
+def test_tracer_past_the_end_reads_vs_reference_program():
+    """One step of 2048 particles at 128^2 against the reference program:
+    every particle matches the driver addressing (past-the-end reads 0; the
+    program's 128 KiB arrays are mmap-allocated), the last row, the last
+    column and the corner cell included; the periodic form differs from the
+    program exactly on those."""
+    g = _golden_snap()
+    nx, lx, nu, dt = 128, 2 * PI, 0.01, 1e-3
+    _, w = O.ic2csh(nx, nx, d_in=0.0234375)
+    w, _ = O.ab2csh_steps(w, np.zeros_like(w, order="F"), nx, nu, dt, 1)
+    ux, uy = O.csh_velocity(w, nx, lx, lx)
+    xp, yp = g["s128_start"][:, 0], g["s128_start"][:, 1]
+    p = g["s128_particles_0"]
+    dx = lx / nx
+    edge = (np.trunc(xp / dx) >= nx - 1) | (np.trunc(yp / dx) >= nx - 1)
+    assert np.sum(np.trunc(yp / dx) >= nx - 1) >= 5 and np.sum(np.trunc(xp / dx) >= nx - 1) >= 5
+    x, y = O.tracer_step(ux, uy, xp, yp, lx, lx, dt, periodic=False)
+    assert np.max(np.abs(p[:, 0] - x)) < 1e-12 and np.max(np.abs(p[:, 1] - y)) < 1e-12
+    x, y = O.tracer_step(ux, uy, xp, yp, lx, lx, dt, periodic=True)
+    differs = (np.abs(p[:, 0] - x) > 1e-12) | (np.abs(p[:, 1] - y) > 1e-12)
+    assert np.array_equal(differs, edge)
