@@ -5,7 +5,11 @@
  * Fortran layout: complex(8) (re, im) interleaved, column-major half spectra
  * (Nh, Ny[, Nz]) with Nh = Nx/2 + 1, i.e. element (i, j[, k]) (1-based) at
  * double offset 2*((i-1) + (j-1)*Nh [+ (k-1)*Nh*Ny]).  Real fields are
- * (Nx, Ny[, Nz]) column-major.  Grid sizes must be powers of two, 32..4096.
+ * (Nx, Ny[, Nz]) column-major.  Grid sizes: each entry point states its own.
+ * The transforms and the 2D incompressible drop-ins (derive2csh, derive2css,
+ * rk42css, the ab2 steppers) take any size, as FFTW and the reference do;
+ * powers of two (32..4096 for the 2D drop-ins) run on the fused kernels, other
+ * sizes on mixed-radix transforms and separate passes (slower, same results).
  *
  * Every function returns 0 (TARA_OK) or a negative status; tara_last_error()
  * gives the message (per thread).  Functions taking host pointers are
@@ -42,7 +46,7 @@ int tara_version(void);
  * x pass last, reading only the real part of the kx = 0 and kx = Nx/2 entries
  * (lib/fftw/rdft/rank-geq2-rdft2.c:40-68).  Unlike FFTW, c2r leaves `in` intact.
  */
-int tara_fft_r2c_1d(int nx, const double* in, double* out);   /* nx 16..8192 */
+int tara_fft_r2c_1d(int nx, const double* in, double* out);   /* any sizes >= 1, at most 2^31 points */
 int tara_fft_c2r_1d(int nx, const double* in, double* out);
 int tara_fft_r2c_2d(int nx, int ny, const double* in, double* out);
 int tara_fft_c2r_2d(int nx, int ny, const double* in, double* out);

@@ -9,6 +9,8 @@
 
 #include "common.h"
 #include "csh2d.h"
+#include "fft_api.h"
+#include "generic2d.h"
 #include "layout.h"
 
 namespace tara {
@@ -94,10 +96,37 @@ struct Ab2Bufs {
 };
 Ab2Bufs g_ab2;
 
+// Grids the fused passes do not take: upload to pitch spec_ld(nx), the
+// any-size RHS (or RK4 step), download.
+int generic_dropin(int nx, int ny, int nh, const CshPhys& ph, const double* in, double* out, bool rk4, double dt)
+{
+    TARA_CHECK(nx >= 2 && ny >= 1, TARA_ERR_UNSUPPORTED, "grid %dx%d unsupported", nx, ny);
+    const int ld = spec_ld(nx);
+    const size_t bytes = (size_t)ld * ny * sizeof(double2);
+    DevBuf a, b;
+    int rc;
+    if ((rc = a.alloc(bytes)) || (rc = b.alloc(bytes))) return rc;
+    TARA_HIP(hipMemset(a.p, 0, bytes));
+    TARA_HIP(hipMemcpy2D(a.p, ld * sizeof(double2), in, nh * sizeof(double2), nh * sizeof(double2), ny,
+                         hipMemcpyHostToDevice));
+    double2* res = a.as<double2>();
+    if (rk4)
+        rc = gen_csh_rk4(nx, ny, ph, dt, res, nullptr);
+    else {
+        rc = gen_csh_rhs(nx, ny, ph, res, b.as<double2>(), nullptr);
+        res = b.as<double2>();
+    }
+    if (rc) return rc;
+    TARA_HIP(hipMemcpy2D(out, nh * sizeof(double2), res, ld * sizeof(double2), nh * sizeof(double2), ny,
+                         hipMemcpyDeviceToHost));
+    return TARA_OK;
+}
+
 int derive_dropin(const char* name, int nx, int ny, int nh, const CshPhys& ph, const double* omegak, double* dt_new)
 {
     TARA_CHECK(omegak && dt_new, TARA_ERR_ARG, "%s: null array", name);
     TARA_CHECK(nh == nx / 2 + 1, TARA_ERR_ARG, "%s: Nh=%d but Nx/2+1=%d", name, nh, nx / 2 + 1);
+    if (!csh_fast_grid(nx, ny)) return generic_dropin(nx, ny, nh, ph, omegak, dt_new, false, 0.0);
     std::lock_guard<std::mutex> lk(g_mu);
     DropIn* d = nullptr;
     int rc = get_dropin(nx, ny, &d);
@@ -120,6 +149,7 @@ int rk4_dropin(const char* name, int nx, int ny, int nh, const CshPhys& ph, doub
 {
     TARA_CHECK(omegak && omegak_new, TARA_ERR_ARG, "%s: null array", name);
     TARA_CHECK(nh == nx / 2 + 1, TARA_ERR_ARG, "%s: Nh=%d but Nx/2+1=%d", name, nh, nx / 2 + 1);
+    if (!csh_fast_grid(nx, ny)) return generic_dropin(nx, ny, nh, ph, omegak, omegak_new, true, dt);
     std::lock_guard<std::mutex> lk(g_mu);
     DropIn* d = nullptr;
     int rc = get_dropin(nx, ny, &d);

@@ -8,6 +8,7 @@
 #include <tuple>
 
 #include "fft_api.h"
+#include "generic3d.h"
 #include "mhd3d.h"
 #include "philox.h"
 
@@ -78,6 +79,7 @@ extern "C" int tara_derive3com(int nx, int ny, int nz, int nh, const tara_mhd_pa
     for (int f = 0; f < 8; ++f) TARA_CHECK(state[f] && rhs[f], TARA_ERR_ARG, "derive3com: null array %d", f);
     int rc = check_dims(nx, ny, nz, nh);
     if (rc) return rc;
+    if (!mhd_fast_grid(nx, ny, nz)) return gen_derive3com_host(nx, ny, nz, prm, state, rhs);
     std::lock_guard<std::mutex> lk(g_mu);
     DropIn* d = nullptr;
     if ((rc = get_dropin(nx, ny, nz, prm, &d))) return rc;

@@ -73,8 +73,12 @@ struct DevBuf {
 //   twiddles(n, rmax): Stockham per-pass table for a length-n complex FFT whose
 //                      schedule caps the radix at rmax (8 or 4; fft_core.h)
 //   rc_twiddles(n)   : e^{-2 pi i k/n}, k = 0..n/2 (real <-> half-length complex packing)
+//   unit_roots(n)    : e^{-2 pi i k/n}, k = 0..n-1 (any n; the mixed-radix FFT)
 const double2* twiddles(int n, int rmax = 8);
 const double2* rc_twiddles(int n);
+const double2* unit_roots(int n);
+// e^{-2 pi i num/den} in long double, rounded once (exact at multiples of pi/4).
+double2 unit_root(long num, long den);
 
 // Host-side table generators.
 void host_twiddles(int n, int rmax, std::vector<double2>& out);

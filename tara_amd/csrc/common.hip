@@ -49,7 +49,7 @@ int current_device()
 
 // e^{-2 pi i num/den} from long-double libm, reduced to the first octant so the
 // argument passed to sinl/cosl is small.
-static double2 unit_root(long num, long den)
+double2 unit_root(long num, long den)
 {
     num %= den;
     if (num < 0) num += den;
@@ -93,7 +93,8 @@ std::mutex g_tab_mu;
 std::map<std::tuple<int, int, int>, void*> g_tabs;  // (device, kind, n) -> device ptr
 
 // kind: 0 = Stockham table with radix cap 8, 1 = real<->complex packing roots,
-// 4 = Stockham table with radix cap 4, 16 = Stockham table with radix cap 16.
+// 2 = all n roots e^{-2 pi i k/n}, 4 = Stockham table with radix cap 4,
+// 16 = Stockham table with radix cap 16.
 const double2* cached_table(int kind, int n)
 {
     int dev = current_device();
@@ -104,7 +105,10 @@ const double2* cached_table(int kind, int n)
     std::vector<double2> h;
     if (kind == 1)
         host_rc_twiddles(n, h);
-    else
+    else if (kind == 2) {
+        h.resize(n);
+        for (int k = 0; k < n; ++k) h[k] = unit_root(k, n);
+    } else
         host_twiddles(n, kind == 4 ? 4 : kind == 16 ? 16 : 8, h);
     void* d = nullptr;
     if (hipMalloc(&d, h.size() * sizeof(double2)) != hipSuccess) return nullptr;
@@ -119,6 +123,7 @@ const double2* cached_table(int kind, int n)
 
 const double2* twiddles(int n, int rmax) { return cached_table(rmax == 4 ? 4 : rmax == 16 ? 16 : 0, n); }
 const double2* rc_twiddles(int n) { return cached_table(1, n); }
+const double2* unit_roots(int n) { return cached_table(2, n); }
 
 }  // namespace tara
 

@@ -125,8 +125,23 @@ static int col_fft_n(double2* a, int ld, long es, long pstride, int nplanes, int
     return TARA_OK;
 }
 
+// Lengths and batch shapes the register-resident kernels take; anything else
+// goes to the mixed-radix transforms of fft_any.hip.
+static bool fast_col(int n, int ld)
+{
+    const int tc = n >= 4096 ? 2 : n >= 2048 ? 4 : 8;
+    return is_pow2(n) && n >= 8 && n <= 4096 && ld % tc == 0;
+}
+
+static bool fast_row(int nx, int nrows)
+{
+    const int m = nx / 2, rb = m >= 2048 ? 1 : m >= 1024 ? 2 : m >= 512 ? 4 : 8;
+    return nx % 2 == 0 && is_pow2(m) && m >= 8 && m <= 4096 && nrows % rb == 0;
+}
+
 int col_fft(double2* a, int n, int ld, long es, long pstride, int nplanes, int dir, hipStream_t st)
 {
+    if (!fast_col(n, ld)) return any_col_fft(a, n, ld, es, pstride, nplanes, dir, st);
     TARA_LEN_SWITCH(n, col_fft_n, a, ld, es, pstride, nplanes, dir, st)
 }
 
@@ -144,6 +159,7 @@ static int row_r2c_n(const double* x, double2* X, int ld, int nrows, hipStream_t
 
 int row_r2c(const double* x, double2* X, int nx, int ld, int nrows, hipStream_t st)
 {
+    if (!fast_row(nx, nrows)) return any_row_r2c(x, X, nx, ld, nrows, st);
     TARA_LEN_SWITCH(nx / 2, row_r2c_n, x, X, ld, nrows, st)
 }
 
@@ -161,6 +177,7 @@ static int row_c2r_n(const double2* X, double* x, int ld, int nrows, double s, h
 
 int row_c2r(const double2* X, double* x, int nx, int ld, int nrows, double s, hipStream_t st)
 {
+    if (!fast_row(nx, nrows)) return any_row_c2r(X, x, nx, ld, nrows, s, st);
     TARA_LEN_SWITCH(nx / 2, row_c2r_n, X, x, ld, nrows, s, st)
 }
 
@@ -173,17 +190,14 @@ using namespace tara;
 namespace {
 int check_dims(int nx, int ny, int nz)
 {
-    const bool one_d = ny == 1 && nz == 1;
-    TARA_CHECK(is_pow2(nx) && nx >= 16 && nx <= 8192 &&
-                   (one_d || (is_pow2(ny) && (nz == 1 || is_pow2(nz)) && ny >= 8 && ny <= 4096 && nz <= 4096)),
-               TARA_ERR_UNSUPPORTED,
-               "transform size %dx%dx%d unsupported (powers of two; nx 16..8192, ny, nz 8..4096 or both 1)", nx, ny,
-               nz);
+    TARA_CHECK(nx >= 1 && ny >= 1 && nz >= 1 && (double)nx * ny * nz <= (double)(1L << 31), TARA_ERR_UNSUPPORTED,
+               "transform size %dx%dx%d unsupported (each >= 1, at most 2^31 points)", nx, ny, nz);
     return TARA_OK;
 }
 
 // Host-array transform: copy in (host layout pitch nh -> device pitch ld), run, copy out.
 // A 1-D transform runs as row 0 of a block of 8 rows (the row kernels' unit).
+// Any sizes: lengths the fast kernels do not take run on the mixed-radix path.
 int fft_host(int nx, int ny, int nz, const double* in, double* out, bool forward)
 {
     int rc = check_dims(nx, ny, nz);

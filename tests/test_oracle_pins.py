@@ -403,3 +403,30 @@ def test_tracer_past_the_end_reads_vs_reference_program():
     x, y = O.tracer_step(ux, uy, xp, yp, lx, lx, dt, periodic=True)
     differs = (np.abs(p[:, 0] - x) > 1e-12) | (np.abs(p[:, 1] - y) > 1e-12)
     assert np.array_equal(differs, edge)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build not present")
+@pytest.mark.parametrize("nx,ny", [(48, 40), (30, 34), (96, 20)])
+def test_derive2csh_oracle_any_grid_vs_reference(nx, ny):
+    """The oracle's derive2csh on grids that are not powers of two (its
+    transforms fall back to the direct DFT) against the reference routine
+    with FFTW, on a full-spectrum random field."""
+    rng = np.random.default_rng(nx * ny)
+    w = O.r2c(np.asfortranarray(rng.standard_normal((nx, ny))))
+    assert rel_l2(O.derive2csh(w, nx, 1e-3), O.ref_derive2csh(w, nx, 1e-3)) < 1e-12
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build not present")
+@pytest.mark.parametrize("shape", [(12, 10, 8), (6, 12, 10)])
+def test_derive3com_oracle_any_grid_vs_reference(shape):
+    """The oracle's derive3com on grids that are not powers of two against
+    the reference routine with FFTW (its 1/450 diffusion)."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx * ny * nz)
+    reals = [1.0 + 0.05 * rng.standard_normal(shape)] + [0.1 * rng.standard_normal(shape) for _ in range(3)]
+    reals += [np.ones(shape)] + [0.1 * rng.standard_normal(shape) for _ in range(3)]
+    st = [O.r2c(np.asfortranarray(r)) for r in reals]
+    got = O.derive3com(st, nx, L, L, L, 1.0 / 450.0, 1.0 / 450.0)
+    want = O.ref_derive3com(st, nx, L, L, L)
+    for f in (0, 1, 2, 3, 5, 6, 7):
+        assert rel_l2(got[f], want[f]) < 1e-12, f
