@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "fam2d.h"
+#include "generic_fam2d.h"
 
 namespace tara {
 namespace {
@@ -128,6 +129,7 @@ extern "C" int tara_derive2coh(int nx, int ny, int nh, const tara_coh_params* pr
     for (int f = 0; f < 3; ++f) TARA_CHECK(state[f] && rhs[f], TARA_ERR_ARG, "derive2coh: null array %d", f);
     int rc = check_nh("derive2coh", nx, nh);
     if (rc) return rc;
+    if (!f2_fast_grid(nx, ny)) return gen_derive2coh_host(nx, ny, prm, state, rhs);
     CohPhys ph;
     ph.mux = prm->mux;
     ph.muy = prm->muy;
@@ -192,6 +194,7 @@ extern "C" int tara_derive2com(int nx, int ny, int nh, const tara_com_params* pr
     for (int f = 0; f < 6; ++f) TARA_CHECK(state[f] && rhs[f], TARA_ERR_ARG, "derive2com: null array %d", f);
     int rc = check_nh("derive2com", nx, nh);
     if (rc) return rc;
+    if (!f2_fast_grid(nx, ny)) return gen_derive2com_host(nx, ny, prm, state, rhs);
     ComPhys ph;
     ph.mu = prm->mu;
     ph.eta = prm->eta;

@@ -128,3 +128,35 @@ def test_derive3com_any_grid_vs_oracle(shape):
     for f in (0, 1, 2, 3, 5, 6, 7):
         assert rel_l2(got[f], ref[f]) < 1e-12, f
     assert np.all(got[4] == 0)
+
+
+COH = dict(pi=np.pi, lx=2 * np.pi, ly=3 * np.pi, mux=2e-3, muy=3e-3, cs=3.77)
+COM = dict(pi=np.pi, lx=2 * np.pi, ly=3 * np.pi, mu=1e-3, eta=2e-3, cs=2.0)
+
+
+@pytest.mark.parametrize("nx,ny", [(96, 60), (24, 30), (64, 48), (40, 512)])
+def test_derive2coh_any_grid_vs_oracle(nx, ny):
+    print(f"derive2coh {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(nx * ny)
+    reals = (1.0 + 0.02 * rng.standard_normal((nx, ny)), 0.1 * rng.standard_normal((nx, ny)),
+             0.1 * rng.standard_normal((nx, ny)))
+    st = [O.r2c(np.asfortranarray(a)) for a in reals]
+    prm = T.CohParams(COH["pi"], COH["lx"], COH["ly"], COH["mux"], COH["muy"], COH["cs"], 0.0)
+    got = T.derive2coh(nx, ny, nx // 2 + 1, prm, st)
+    ref = O.derive2coh(st, nx, **COH)
+    for k in range(3):
+        assert rel_l2(got[k], ref[k]) < 1e-12, k
+
+
+@pytest.mark.parametrize("nx,ny", [(96, 60), (24, 30), (64, 48)])
+def test_derive2com_any_grid_with_forcing_vs_oracle(nx, ny):
+    print(f"derive2com {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(nx + 7 * ny)
+    reals = [1.0 + 0.02 * rng.standard_normal((nx, ny))] + [0.1 * rng.standard_normal((nx, ny)) for _ in range(2)]
+    reals += [2.0 + 0.05 * rng.standard_normal((nx, ny))] + [0.1 * rng.standard_normal((nx, ny)) for _ in range(2)]
+    st = [O.r2c(np.asfortranarray(a)) for a in reals]
+    prm = T.ComParams(COM["pi"], COM["lx"], COM["ly"], COM["mu"], COM["eta"], COM["cs"], 1.0, 0.3, 0.0)
+    got = T.derive2com(nx, ny, nx // 2 + 1, prm, st)
+    ref = O.derive2com(st, nx, kf=1.0, a=0.3, **COM)
+    for k in range(6):
+        assert rel_l2(got[k], ref[k]) < 1e-12, k

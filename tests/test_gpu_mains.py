@@ -79,12 +79,19 @@ B0 = 0.0d0
 COLUMNS = ["time", "Pressure", "Energy", "T_Energy", "B_Field", "C1", "C2", "Hf", "Hm", "HT", "div_B_Tot"]
 
 
+def _unlimited_stack():
+    """derive3com keeps ~130 automatic arrays on the stack (29 MB at 24^3)."""
+    import resource
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
 def _run(exe, where, n, steps):
     where.mkdir()
     (where / "output").mkdir()
     (where / "input.ini").write_text(INI.format(n=n, tmax=repr((steps - 0.5) * 1e-3)))
     env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="4096")
-    r = subprocess.run([exe, "input.ini"], cwd=where, env=env, capture_output=True, text=True, timeout=600)
+    r = subprocess.run([exe, "input.ini"], cwd=where, env=env, capture_output=True, text=True, timeout=600,
+                       preexec_fn=_unlimited_stack)
     assert r.returncode == 0, (exe, r.stdout[-2000:], r.stderr[-2000:])
     text = (where / "output" / "Energy.dat").read_text().replace("D", "E")
     vals = np.array([float(v) for v in text.split()])
@@ -92,7 +99,7 @@ def _run(exe, where, n, steps):
     return vals.reshape(steps, len(COLUMNS))
 
 
-@pytest.mark.parametrize("n,steps", [(16, 20)])
+@pytest.mark.parametrize("n,steps", [(16, 20), (24, 6)])
 def test_reference_main3com_through_tara_vs_reference_program(tmp_path, n, steps):
     print(f"main3com {n}^3 x {steps}: reference program (CPU)", flush=True)
     want = _run(TARA_REF, tmp_path / "ref", n, steps)
@@ -170,3 +177,36 @@ def test_reference_main2csh_through_tara_vs_reference_program(tmp_path):
         err = np.max(np.abs(p[ok] - want[ok]))
         print(f"  t={t} particles: max diff {err:.2e}", flush=True)
         assert err < 1e-10, (t, err)
+
+
+@pytest.mark.skipif(not (os.path.exists(TARA_SNAP) and os.path.exists(TARA_SNAP + "_ref")),
+                    reason="config-1 programs not linked")
+def test_reference_main2csh_any_grid_through_tara(tmp_path):
+    """The config-1 program on a 96 x 80 grid (not powers of two: derive2csh
+    and the main's own c2r through the mixed-radix path), one loop trip,
+    against the program built against the reference routines and FFTW: the
+    t = 0 snapshot (fields, and the particles that do not start in the last
+    row, where the program reads past its arrays into the heap)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_golden as G
+
+    ini = G.snap_ini(96, 256, 1).replace("Ny = 96", "Ny = 80")
+    out = {}
+    for exe, name in ((TARA_SNAP + "_ref", "ref"), (TARA_SNAP, "tara")):
+        d = tmp_path / name
+        (d / "output").mkdir(parents=True)
+        (d / "input.ini").write_text(ini)
+        print(f"main2csh 96x80 x 1: {name}", flush=True)
+        r = subprocess.run([exe, "input.ini"], cwd=d, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, (name, r.stdout[-2000:], r.stderr[-2000:])
+        out[name] = (_read_fort(d / "output" / "fort.00000100", 5), _read_fort(d / "output" / "fort.00000105", 2))
+    (fr, pr), (ft, pt) = out["ref"], out["tara"]
+    assert fr.shape == (96 * 80, 5) and np.array_equal(ft[:, :2], fr[:, :2])
+    for c in (2, 3, 4):
+        err = np.max(np.abs(ft[:, c] - fr[:, c])) / np.max(np.abs(fr[:, c]))
+        print(f"  column {c}: max rel diff {err:.2e}", flush=True)
+        assert err < 1e-10, (c, err)
+    start = G.rand_starts(256, 2 * np.pi)
+    ok = np.trunc(start[:, 1] / (2 * np.pi / 80)) < 79
+    assert np.max(np.abs(pt[ok] - pr[ok])) < 1e-10

@@ -430,3 +430,26 @@ def test_derive3com_oracle_any_grid_vs_reference(shape):
     want = O.ref_derive3com(st, nx, L, L, L)
     for f in (0, 1, 2, 3, 5, 6, 7):
         assert rel_l2(got[f], want[f]) < 1e-12, f
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build not present")
+@pytest.mark.parametrize("nx,ny", [(24, 30), (20, 12)])
+def test_derive2coh_derive2com_oracle_any_grid_vs_reference(nx, ny):
+    """The oracle's derive2coh and derive2com (forcing off: the reference's
+    forcing reads uninitialised coordinates) on grids that are not powers of
+    two against the reference routines with FFTW."""
+    rng = np.random.default_rng(nx + ny)
+    coh = dict(pi=PI, lx=2 * PI, ly=3 * PI, mux=2e-3, muy=3e-3, cs=3.77)
+    st = [O.r2c(np.asfortranarray(a)) for a in (1.0 + 0.02 * rng.standard_normal((nx, ny)),
+                                                 0.1 * rng.standard_normal((nx, ny)),
+                                                 0.1 * rng.standard_normal((nx, ny)))]
+    got, want = O.derive2coh(st, nx, **coh), O.ref_derive2coh(st, nx, **coh)
+    for k in range(3):
+        assert rel_l2(got[k], want[k]) < 1e-12, ("coh", k)
+    com = dict(pi=PI, lx=2 * PI, ly=3 * PI, mu=1e-3, eta=2e-3, cs=2.0)
+    reals = [1.0 + 0.02 * rng.standard_normal((nx, ny))] + [0.1 * rng.standard_normal((nx, ny)) for _ in range(2)]
+    reals += [2.0 + 0.05 * rng.standard_normal((nx, ny))] + [0.1 * rng.standard_normal((nx, ny)) for _ in range(2)]
+    st = [O.r2c(np.asfortranarray(a)) for a in reals]
+    got, want = O.derive2com(st, nx, kf=0.0, a=0.0, **com), O.ref_derive2com(st, nx, **com)
+    for k in range(6):
+        assert rel_l2(got[k], want[k]) < 1e-12, ("com", k)
