@@ -278,7 +278,9 @@ int tara_ab2com(int nx, int ny, int nh, double dt, const double* const state[6],
  * loops of src/main2coh.f95:278-300 and src/main2com.f95:364-380, without the
  * diagnostics).  upload/download take 3 (coh) or 6 (com) arrays; the AB2
  * history starts at zero on upload.  Timing classes: 0 = prep + inverse y,
- * 1 = x passes (+ the com middle pass), 2 = forward y + assembly. */
+ * 1 = x passes (+ the com middle pass), 2 = forward y + assembly.  sync and
+ * download return TARA_ERR_NONFINITE once an update has stored a NaN or Inf
+ * (the flag clears on upload). */
 typedef struct tara_f2 tara_f2;
 int tara_coh_create(int nx, int ny, const tara_coh_params* p, tara_f2** out);
 int tara_com_create(int nx, int ny, const tara_com_params* p, tara_f2** out);

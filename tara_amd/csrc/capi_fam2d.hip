@@ -278,6 +278,16 @@ extern "C" int tara_com_create(int nx, int ny, const tara_com_params* prm, tara_
     return TARA_OK;
 }
 
+// Reads the context's non-finite flag (after its stream has drained).
+static int f2_check_finite(tara_f2* c)
+{
+    int bad = 0;
+    TARA_HIP(hipMemcpyAsync(&bad, c->plan.bad.p, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    TARA_HIP(hipStreamSynchronize(c->s));
+    TARA_CHECK(bad == 0, TARA_ERR_NONFINITE, "the state became non-finite (NaN or Inf) during advance");
+    return TARA_OK;
+}
+
 extern "C" int tara_f2_upload(tara_f2* c, const double* const state[])
 {
     TARA_CHECK(c && state, TARA_ERR_ARG, "null argument");
@@ -287,6 +297,7 @@ extern "C" int tara_f2_upload(tara_f2* c, const double* const state[])
         if (rc) return rc;
         TARA_HIP(hipMemsetAsync(c->hist[f].p, 0, c->hist[f].bytes, c->s));
     }
+    TARA_HIP(hipMemsetAsync(c->plan.bad.p, 0, sizeof(int), c->s));
     TARA_HIP(hipStreamSynchronize(c->s));
     return TARA_OK;
 }
@@ -299,8 +310,7 @@ extern "C" int tara_f2_download(tara_f2* c, double* const state[])
         int rc = d2h(c->plan, state[f], c->st[f].as<double2>(), c->s);
         if (rc) return rc;
     }
-    TARA_HIP(hipStreamSynchronize(c->s));
-    return TARA_OK;
+    return f2_check_finite(c);
 }
 
 extern "C" int tara_f2_advance(tara_f2* c, int nsteps)
@@ -323,8 +333,7 @@ extern "C" int tara_f2_advance(tara_f2* c, int nsteps)
 extern "C" int tara_f2_sync(tara_f2* c)
 {
     TARA_CHECK(c, TARA_ERR_ARG, "null context");
-    TARA_HIP(hipStreamSynchronize(c->s));
-    return TARA_OK;
+    return f2_check_finite(c);
 }
 
 extern "C" int tara_f2_destroy(tara_f2* c)

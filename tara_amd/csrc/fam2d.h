@@ -20,6 +20,7 @@ struct F2Plan {
     DevBuf b;                  // products after the x pass
     DevBuf c, d, k;            // com only: first-round x output, derivative fields, u / forcing spectra
     DevBuf kx, ky, imin;       // imin[j]: modes with i >= imin[j] are dealiased away
+    DevBuf bad;                // one int: set when an AB2 update stores a NaN or Inf
     const double2* tw_y = nullptr;
     const double2* tw_x = nullptr;  // length nx, radix cap 4
 };

@@ -381,6 +381,7 @@ struct F2YArgs {
     const int* imin;
     int ld;
     double dt, mu, eta;
+    int* bad;           // set when an AB2 update stores a non-finite value
 };
 
 #define F2Y_PROLOGUE(NY)                                                      \
@@ -412,13 +413,15 @@ __device__ __forceinline__ void f2_yfft_fwd(const double2* src, int i, int ld, i
 }
 
 template <int MODE>
-__device__ __forceinline__ void f2_finish(double2* st, double2* out, size_t q, double2 f, double dt)
+__device__ __forceinline__ void f2_finish(double2* st, double2* out, size_t q, double2 f, double dt, int* bad)
 {
     if (MODE == F2_RHS) {
         out[q] = f;
     } else {
         const double2 fo = out[q], w = st[q];
-        st[q] = make_double2(w.x + (1.5 * f.x - 0.5 * fo.x) * dt, w.y + (1.5 * f.y - 0.5 * fo.y) * dt);
+        const double2 nw = make_double2(w.x + (1.5 * f.x - 0.5 * fo.x) * dt, w.y + (1.5 * f.y - 0.5 * fo.y) * dt);
+        if (!(isfinite(nw.x) && isfinite(nw.y))) *bad = 1;
+        st[q] = nw;
         out[q] = f;
     }
 }
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(CShape<NY>::NT) void k_coh_yfwd(F2YArgs a, const do
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const double2 r = keep[e] ? v[e] : make_double2(0.0, 0.0);
-            f2_finish<MODE>(a.st[f], a.out[f], q[e], make_double2(-r.x, -r.y), a.dt);
+            f2_finish<MODE>(a.st[f], a.out[f], q[e], make_double2(-r.x, -r.y), a.dt, a.bad);
         }
     }
 }
@@ -479,7 +482,7 @@ __global__ __launch_bounds__(CShape<NY>::NT) void k_com_yfwd(F2YArgs a, const do
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const double2 s1 = msk(e, f2_ik(kxv, a.st[1][q[e]])), s2 = msk(e, f2_ik(kyv[e], a.st[2][q[e]]));
-        f2_finish<MODE>(a.st[0], a.out[0], q[e], make_double2(-(s1.x + s2.x), -(s1.y + s2.y)), a.dt);
+        f2_finish<MODE>(a.st[0], a.out[0], q[e], make_double2(-(s1.x + s2.x), -(s1.y + s2.y)), a.dt, a.bad);
     }
     // rho ux: -(i kx Mom_x_1 + i ky Mom_x_2) - mu (kx^2 ux_k + ky^2 ux_k) + Fx_k
     f2_yfft_fwd<NY>(a.b, i, a.ld, t, xl, tw, v);  // Mom_x_1
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(CShape<NY>::NT) void k_com_yfwd(F2YArgs a, const do
         const double2 k1 = msk(e, f2_k2(kxv, u)), k2 = msk(e, f2_k2(kyv[e], u));
         double2 r = make_double2(-(acc[e].x + d.x), -(acc[e].y + d.y));
         r = make_double2((r.x - a.mu * (k1.x + k2.x)) + fx.x, (r.y - a.mu * (k1.y + k2.y)) + fx.y);
-        f2_finish<MODE>(a.st[1], a.out[1], q[e], r, a.dt);
+        f2_finish<MODE>(a.st[1], a.out[1], q[e], r, a.dt, a.bad);
     }
     // rho uy: -(i kx Mom_y_1 + i ky Mom_y_2) - mu (kx^2 uy_k + ky^2 uy_k) + Fy_k
     f2_yfft_fwd<NY>(a.b + 2 * a.fs, i, a.ld, t, xl, tw, v);  // Mom_y_2
@@ -505,7 +508,7 @@ __global__ __launch_bounds__(CShape<NY>::NT) void k_com_yfwd(F2YArgs a, const do
         const double2 k1 = msk(e, f2_k2(kxv, u)), k2 = msk(e, f2_k2(kyv[e], u));
         double2 r = make_double2(-(acc2[e].x + d.x), -(acc2[e].y + d.y));
         r = make_double2((r.x - a.mu * (k1.x + k2.x)) + fy.x, (r.y - a.mu * (k1.y + k2.y)) + fy.y);
-        f2_finish<MODE>(a.st[2], a.out[2], q[e], r, a.dt);
+        f2_finish<MODE>(a.st[2], a.out[2], q[e], r, a.dt, a.bad);
     }
     // E: -(i kx Energy_x + i ky Energy_y) + mu E_Visc
     f2_yfft_fwd<NY>(a.b + 3 * a.fs, i, a.ld, t, xl, tw, v);
@@ -521,7 +524,7 @@ __global__ __launch_bounds__(CShape<NY>::NT) void k_com_yfwd(F2YArgs a, const do
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const double2 ev = msk(e, v[e]);
-        f2_finish<MODE>(a.st[3], a.out[3], q[e], make_double2(acc[e].x + a.mu * ev.x, acc[e].y + a.mu * ev.y), a.dt);
+        f2_finish<MODE>(a.st[3], a.out[3], q[e], make_double2(acc[e].x + a.mu * ev.x, acc[e].y + a.mu * ev.y), a.dt, a.bad);
     }
     // Bx: -i ky Mag_x - eta (kx^2 + ky^2) Bx;  By: -i kx Mag_y - eta (...) By, Mag_y = -Mag_x
     f2_yfft_fwd<NY>(a.b + 6 * a.fs, i, a.ld, t, xl, tw, v);
@@ -534,8 +537,8 @@ __global__ __launch_bounds__(CShape<NY>::NT) void k_com_yfwd(F2YArgs a, const do
         const double2 c1 = msk(e, f2_k2(kxv, byk)), c2 = msk(e, f2_k2(kyv[e], byk));
         const double2 rbx = make_double2(-mx.x - a.eta * (b1.x + b2.x), -mx.y - a.eta * (b1.y + b2.y));
         const double2 rby = make_double2(-my.x - a.eta * (c1.x + c2.x), -my.y - a.eta * (c1.y + c2.y));
-        f2_finish<MODE>(a.st[4], a.out[4], q[e], rbx, a.dt);
-        f2_finish<MODE>(a.st[5], a.out[5], q[e], rby, a.dt);
+        f2_finish<MODE>(a.st[4], a.out[4], q[e], rbx, a.dt, a.bad);
+        f2_finish<MODE>(a.st[5], a.out[5], q[e], rby, a.dt, a.bad);
     }
 }
 
@@ -671,8 +674,9 @@ int f2_plan_init(F2Plan& p, int family, int nx, int ny, double pi, double lx, do
         imin[j] = m;  // kx increases with i: every i >= m is cut
     }
     if ((rc = p.kx.alloc(p.ld * sizeof(double))) || (rc = p.ky.alloc(ny * sizeof(double))) ||
-        (rc = p.imin.alloc(ny * sizeof(int))))
+        (rc = p.imin.alloc(ny * sizeof(int))) || (rc = p.bad.alloc(sizeof(int))))
         return rc;
+    TARA_HIP(hipMemset(p.bad.p, 0, sizeof(int)));
     TARA_HIP(hipMemcpy(p.kx.p, kx.data(), p.ld * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.ky.p, ky.data(), ny * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.imin.p, imin.data(), ny * sizeof(int), hipMemcpyHostToDevice));
@@ -728,6 +732,7 @@ int coh_rhs_stage(const F2Plan& p, const CohPhys& ph, double2* const st[3], int 
     y.imin = p.imin.as<int>();
     y.ld = p.ld;
     y.dt = dt;
+    y.bad = p.bad.as<int>();
     if (tm) tm->begin(2, s);
     if ((rc = ypass(p, 0, mode, y, s))) return rc;
     if (tm) tm->end(2, s);
@@ -784,6 +789,7 @@ int com_rhs_stage(const F2Plan& p, const ComPhys& ph, double2* const st[6], int 
     y.dt = dt;
     y.mu = ph.mu;
     y.eta = ph.eta;
+    y.bad = p.bad.as<int>();
     if ((rc = ypass(p, 1, mode, y, s))) return rc;
     x.b = p.b.as<double2>();
     x.d = p.d.as<double2>();

@@ -3,8 +3,9 @@ reference, which never checks its state).  The condition is forced from the
 host: a state with one NaN (or Inf) mode is uploaded, every kernel runs to its
 normal end (NaN arithmetic faults nothing), and the sync reports it.
 
-  * 2D and 3D: sync and download return TARA_ERR_NONFINITE after the step;
-    the flag clears when a new state is uploaded.
+  * 2D, 2D compressible and 3D: sync and download return
+    TARA_ERR_NONFINITE after the step; the flag clears when a new state is
+    uploaded.
   * Ensemble: only the poisoned run is flagged, and the other runs of its
     bucket and of the other buckets match the same runs done alone.
 """
@@ -91,3 +92,33 @@ def test_ensemble_guard_isolates_the_failed_run():
     e.sync()
     assert e.nonfinite() == [False] * 4
     e.close()
+
+
+@pytest.mark.parametrize("family", ["coh", "com"])
+def test_f2_guard(family):
+    print(f"guard {family} 64^2", flush=True)
+    n, L = 64, 2 * np.pi
+    rng = np.random.default_rng(5)
+    if family == "coh":
+        prm = T.CohParams(np.pi, L, L, 2e-4, 2e-4, 3.77, 1e-4)
+        reals = [1.0 + 0.02 * rng.standard_normal((n, n))] + [0.05 * rng.standard_normal((n, n)) for _ in range(2)]
+    else:
+        prm = T.ComParams(np.pi, L, L, 1e-3, 1e-4, 2.0, 0.0, 0.0, 1e-4)
+        reals = [1.0 + 0.02 * rng.standard_normal((n, n))] + [0.05 * rng.standard_normal((n, n)) for _ in range(2)]
+        reals += [2.0 + 0.02 * rng.standard_normal((n, n))] + [0.05 * rng.standard_normal((n, n)) for _ in range(2)]
+    st = [T.fft_r2c(np.asfortranarray(a)) for a in reals]
+    s = T.F2Solver(n, n, prm)
+    s.upload(st)
+    s.advance(3)
+    s.sync()
+    bad = [a.copy() for a in st]
+    bad[1][4, 9] = np.nan
+    s.upload(bad)
+    s.advance(1)
+    _raises_nonfinite(s.sync)
+    _raises_nonfinite(s.download)
+    s.upload(st)
+    s.advance(2)
+    s.sync()
+    assert all(np.all(np.isfinite(a)) for a in s.download())
+    s.close()
