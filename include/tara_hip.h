@@ -401,6 +401,11 @@ int tara_fno_forward(tara_fno* f, int batch, const float* omega, float* out);
 int tara_fno_grads(tara_fno* f, int batch, const float* omega, const float* target, float* loss, float* grad_host);
 int tara_fno_train_step(tara_fno* f, int batch, const float* omega, const float* target, float lr, float* loss);
 int tara_fno_rel_err(tara_fno* f, int batch, const float* a, const float* ref, float* err_host);
+/* Training batches: xb[b] = inp[idx[b]], yb[b] = tgt[idx[b]] (device arrays of
+ * n x n floats per sample; idx a host array of batch indices in [0, npairs),
+ * batch <= 64), enqueued on the network's stream before its next step. */
+int tara_fno_gather_pairs(tara_fno* f, int batch, const long* idx, long npairs, const float* inp, const float* tgt,
+                          float* xb, float* yb);
 /* Surrogate rollout with the FP64 solver as checker.  traj: device float
  * [nsteps+1][batch][n][n], traj[0] set by the caller; step t+1 = network(step t).
  * Every `every`-th step t+1 = (j+1)*every (j < nchk = nsteps / every) is

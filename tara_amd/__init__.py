@@ -123,6 +123,8 @@ def lib():
         L.tara_fno_grads.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _fp, _fp]
         L.tara_fno_train_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, _fp]
         L.tara_fno_rel_err.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _fp]
+        L.tara_fno_gather_pairs.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long), C.c_long, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p]
         L.tara_fno_destroy.argtypes = [C.c_void_p]
         L.tara_fno_rollout.argtypes = ([C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                         C.POINTER(C.c_void_p), C.c_void_p, C.c_int, C.c_float, C.c_int, _fp,
@@ -421,6 +423,14 @@ class FNO:
         loss = C.c_float()
         _check(lib().tara_fno_train_step(self._h, batch, _dptr(omega), _dptr(target), lr, C.byref(loss)))
         return loss.value
+
+    def gather_pairs(self, idx, inp, tgt, xb, yb):
+        """xb[b] = inp[idx[b]], yb[b] = tgt[idx[b]] on the device (inp, tgt:
+        (npairs, n, n) float32 device tensors; idx: host integer array)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        npairs = inp.shape[0]
+        _check(lib().tara_fno_gather_pairs(self._h, idx.size, idx.ctypes.data_as(C.POINTER(C.c_long)), npairs,
+                                           _dptr(inp), _dptr(tgt), _dptr(xb), _dptr(yb)))
 
     def rel_err(self, batch, a, ref):
         out = np.zeros(batch, np.float32)

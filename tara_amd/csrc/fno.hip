@@ -865,6 +865,44 @@ extern "C" int tara_fno_train_step(tara_fno* f, int batch, const float* omega, c
     return refresh_weights(f);
 }
 
+namespace {
+// Training-batch indices passed by value (at most 64 samples per batch).
+struct PairIdx {
+    long i[64];
+};
+
+// xb[b] = inp[idx[b]], yb[b] = tgt[idx[b]] for b < batch; per floats per sample.
+__global__ void k_gather_pairs(const float* __restrict__ inp, const float* __restrict__ tgt, long per, PairIdx idx,
+                               int batch, float* __restrict__ xb, float* __restrict__ yb)
+{
+    const long total = (long)batch * per;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        const long b = q / per, e = q - b * per;
+        const long src = idx.i[b] * per + e;
+        xb[q] = inp[src];
+        yb[q] = tgt[src];
+    }
+}
+}  // namespace
+
+extern "C" int tara_fno_gather_pairs(tara_fno* f, int batch, const long* idx, long npairs, const float* inp,
+                                     const float* tgt, float* xb, float* yb)
+{
+    TARA_CHECK(f && idx && inp && tgt && xb && yb && batch >= 1 && batch <= f->maxb && batch <= 64 && npairs >= 1,
+               TARA_ERR_ARG, "fno: bad argument");
+    PairIdx pi{};
+    for (int b = 0; b < batch; ++b) {
+        TARA_CHECK(idx[b] >= 0 && idx[b] < npairs, TARA_ERR_ARG, "fno: pair index %ld out of [0, %ld)", idx[b],
+                   npairs);
+        pi.i[b] = idx[b];
+    }
+    const long per = (long)f->n * f->n;
+    hipLaunchKernelGGL(k_gather_pairs, dim3(fno_grid((size_t)batch * per)), dim3(256), 0, f->st, inp, tgt, per, pi,
+                       batch, xb, yb);
+    TARA_HIP(hipGetLastError());
+    return TARA_OK;
+}
+
 extern "C" int tara_fno_rel_err(tara_fno* f, int batch, const float* a, const float* ref, float* err_host)
 {
     TARA_CHECK(f && a && ref && err_host && batch >= 1 && batch <= f->maxb, TARA_ERR_ARG, "fno: bad argument");

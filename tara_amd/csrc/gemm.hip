@@ -4,8 +4,8 @@
 //   C[z](m, n) (=|+=) sum_k A[z](m, k) B[z](k, n),   z = (z1, z2)
 //
 // A(m, k) at A + m*sam + k*sak (+ z1*sa1 + z2*sa2), likewise B and C, so any
-// transpose is a choice of strides.  Operands are bfloat16 (MFMA
-// 32x32x8 bf16, FP32 accumulation) or float (MFMA 32x32x2 f32); the result is
+// transpose is a choice of strides.  Operands are bfloat16 (gfx950's MFMA
+// 32x32x16 bf16, FP32 accumulation) or float (MFMA 32x32x2 f32); the result is
 // stored as float, accumulated into float, or rounded to bfloat16.
 //
 // Tiling: a 256-thread workgroup computes a 64 x 64 tile of C, each wavefront
@@ -20,7 +20,7 @@
 
 namespace tara {
 
-typedef short __attribute__((ext_vector_type(4))) v4s;
+typedef __bf16 __attribute__((ext_vector_type(8))) v8bf;
 typedef float __attribute__((ext_vector_type(16))) v16f;
 
 __device__ __forceinline__ float bf2f(uint16_t h)
@@ -126,8 +126,8 @@ __device__ __forceinline__ void stash_tile(T (*S)[GBK + GPAD], const T* reg)
 template <typename T, int EPI, int AM, int BM>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g, int avec, int bvec)
 {
-    __shared__ T As[GBM][GBK + GPAD];
-    __shared__ T Bs[GBN][GBK + GPAD];  // transposed: Bs[n][k]
+    __shared__ __align__(16) T As[GBM][GBK + GPAD];
+    __shared__ __align__(16) T Bs[GBN][GBK + GPAD];  // transposed: Bs[n][k]
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int z1 = blockIdx.z % g.nb1, z2 = blockIdx.z / g.nb1;
     const T* A = reinterpret_cast<const T*>(g.a) + z1 * g.sa1 + z2 * g.sa2;
@@ -150,17 +150,14 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g, int avec, int bvec)
             fetch_tile<T, BM>(B, g.sbn, g.sbk, n0, k0 + GBK, g.n, g.k, bvec != 0, rb);
         }
         if constexpr (sizeof(T) == 2) {
-            // 32x32x8 bf16: lane l supplies row/col l % 32, k = 4 (l / 32) .. +3
+            // gfx950's 32x32x16 bf16: lane l supplies row/col l % 32, k = 8 (l / 32) .. +7, one 16-byte
+            // LDS read per operand (rows of 144 bytes: 16-byte aligned, 8 lanes on distinct banks)
 #pragma unroll
-            for (int kk = 0; kk < GBK; kk += 8) {
-                const int ko = kk + 4 * (lane / 32);
-                v4s a, b;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    a[q] = (short)As[wm + lane % 32][ko + q];
-                    b[q] = (short)Bs[wn + lane % 32][ko + q];
-                }
-                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+            for (int kk = 0; kk < GBK; kk += 16) {
+                const int ko = kk + 8 * (lane / 32);
+                const v8bf a = *reinterpret_cast<const v8bf*>(&As[wm + lane % 32][ko]);
+                const v8bf b = *reinterpret_cast<const v8bf*>(&Bs[wn + lane % 32][ko]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
             }
         } else {
             // 32x32x2 f32: lane l supplies row/col l % 32, k = l / 32

@@ -62,9 +62,7 @@ def train(net, snaps, epochs, batch, lr=2e-3, lr_min=5e-5, seed=0, log_every=0):
         perm = rng.permutation(npairs)
         acc = 0.0
         for i in range(per_epoch):
-            idx = torch.from_numpy(perm[i * batch:(i + 1) * batch]).cuda()
-            torch.index_select(inp, 0, idx, out=xb)
-            torch.index_select(tgt, 0, idx, out=yb)
+            net.gather_pairs(perm[i * batch:(i + 1) * batch], inp, tgt, xb, yb)
             lr_t = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * it / max(total - 1, 1)))
             loss = net.train_step(batch, xb, yb, lr_t)
             acc += loss

@@ -125,6 +125,24 @@ def test_fno_rel_err():
     assert np.allclose(e, want, rtol=1e-5)
 
 
+def test_fno_gather_pairs():
+    """The training batch gather equals indexing, and an index outside the
+    pair store is refused before anything runs."""
+    n, nb, npairs = 32, 8, 50
+    net = T.FNO(n, 8, 6, 8, 1, max_batch=nb, seed=2)
+    rng = np.random.default_rng(4)
+    snaps = torch.from_numpy(rng.standard_normal((npairs + 1, n, n)).astype(np.float32)).cuda()
+    inp, tgt = snaps[:-1], snaps[1:]
+    xb = torch.empty((nb, n, n), dtype=torch.float32, device="cuda")
+    yb = torch.empty_like(xb)
+    idx = rng.permutation(npairs)[:nb]
+    net.gather_pairs(idx, inp, tgt, xb, yb)
+    torch.cuda.synchronize()
+    assert torch.equal(xb.cpu(), inp.cpu()[idx]) and torch.equal(yb.cpu(), tgt.cpu()[idx])
+    with pytest.raises(T.TaraError):
+        net.gather_pairs(np.array([0, npairs]), inp, tgt, xb, yb)
+
+
 def test_fno_rejects_bad_shapes():
     with pytest.raises(T.TaraError):
         T.FNO(30, 8, 6, 8, 1)  # n not a multiple of 8
