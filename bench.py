@@ -16,6 +16,7 @@ FFTW threads = the host's cores, three AB2 steps at 256^3.
 """
 import argparse
 import json
+import math
 import os
 import resource
 import subprocess
@@ -26,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X vector FP64
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and, in a
 # separate run, WRITE_SIZE over this script (--steps 2 --warmup 1
@@ -377,6 +379,7 @@ def main():
         mhd["cpu_baseline"] = cpu3
     ens = None if a.no_ensemble else bench_ensemble(min(a.steps, 20), a.warmup)
     fno = None if a.no_fno else bench_fno(a.fno_epochs)
+    fft_flops = 4 * 5 * 2.5 * (n * n) * math.log2(n * n)
     out = {
         "metric": METRIC,
         "value": n * n / (ms_per_step * 1e-3),
@@ -401,6 +404,12 @@ def main():
         "xpass_shape": xshape,
         "step_model_bytes": step_bytes,
         "step_model_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        # FP64 work of the step's transforms: 4 RHS x 5 real 2D transforms of
+        # 2.5 N log2 N flops each (N = Nx Ny points), against the vector FP64 peak
+        "fp64_fft_model": {"flops_per_step": fft_flops,
+                           "tflops": round(fft_flops / (ms_per_step * 1e-3) / 1e12, 2),
+                           "peak_tflops": FP64_PEAK_TFLOPS,
+                           "frac": round(fft_flops / (ms_per_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)},
         "state_finite": finite,
         "cpu_baseline": cpu,
         "secondary_2d_1024": sec_1024,
