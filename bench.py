@@ -100,6 +100,23 @@ def cpu_baseline(case, n, steps):
             "sample": sample, "cpu_model": d["cpu_model"], "nproc": d["nproc"], "threads": d["cores"]}
 
 
+def device_copy_gbps(mib=1024):
+    """HBM copy bandwidth (read + write) of a plain device copy of mib MiB per
+    buffer, larger than the last-level cache, from tara_amd/calib/copy_probe
+    in a child process: the practical ceiling beside the 8 TB/s peak."""
+    exe = os.path.join(ROOT, "tara_amd", "calib", "copy_probe")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe, str(mib)], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr[-2000:])
+        return None
+    try:
+        return float(r.stdout.split(" GB/s")[0].split(",")[-1])
+    except (ValueError, IndexError):
+        return None
+
+
 def bench_2d(n, steps, warmup, nu=1e-4, dt=2e-4):
     """Secondary line: the same RK4 step at another grid (the configuration's
     other size, 1024^2): value, per-kernel times and the model rate."""
@@ -336,6 +353,7 @@ def main():
     # CPU baselines first, in child processes, before this process touches the GPU.
     cpu = None if a.no_cpu_baseline else cpu_baseline("2d", n, 3)
     cpu3 = None if (a.no_cpu_baseline or a.no_3d) else cpu_baseline("3d", 256, 3)
+    copy_gbps = device_copy_gbps()
     if not a.no_fno:
         # torch (device memory of the surrogate leg) is loaded before the HIP
         # runtime starts in this process: loaded after, it finds no device.
@@ -399,7 +417,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((names[dom], n, xshape) if dom == 1 else (names[dom], n)),
-                     "algorithmic_bytes_per_launch": model[dom]},
+                     "algorithmic_bytes_per_launch": model[dom],
+                     "device_copy_GBps": copy_gbps,
+                     "frac_of_copy": round(achieved / copy_gbps, 4) if copy_gbps else None},
         "kernels": kernels,
         "xpass_shape": xshape,
         "step_model_bytes": step_bytes,
