@@ -103,7 +103,8 @@ def cpu_baseline(case, n, steps):
 def device_copy_gbps(mib=1024):
     """HBM copy bandwidth (read + write) of a plain device copy of mib MiB per
     buffer, larger than the last-level cache, from tara_amd/calib/copy_probe
-    in a child process: the practical ceiling beside the 8 TB/s peak."""
+    in a child process (the better of its two launch shapes): a practical
+    ceiling beside the 8 TB/s peak."""
     exe = os.path.join(ROOT, "tara_amd", "calib", "copy_probe")
     if not os.path.exists(exe):
         return None
@@ -112,7 +113,7 @@ def device_copy_gbps(mib=1024):
         sys.stderr.write(r.stderr[-2000:])
         return None
     try:
-        return float(r.stdout.split(" GB/s")[0].split(",")[-1])
+        return float(r.stdout.split("best ")[1].split(" GB/s")[0])
     except (ValueError, IndexError):
         return None
 
