@@ -8,6 +8,7 @@
 // These are the building blocks the fused solver passes are made of, exposed
 // on their own for parity testing and for the driver-side transforms.
 #include <cstring>
+#include <memory>
 
 #include "common.h"
 #include "fft_api.h"
@@ -198,19 +199,32 @@ int check_dims(int nx, int ny, int nz)
 // Host-array transform: copy in (host layout pitch nh -> device pitch ld), run, copy out.
 // A 1-D transform runs as row 0 of a block of 8 rows (the row kernels' unit).
 // Any sizes: lengths the fast kernels do not take run on the mixed-radix path.
+// Device work space of the host-array transforms, per device, kept between
+// calls and grown when a larger transform comes (the kept mains call the same
+// sizes every step).
+struct HostWs {
+    DevBuf spec, real;
+};
+std::mutex g_ws_mu;
+std::map<int, std::unique_ptr<HostWs>> g_ws;
+
 int fft_host(int nx, int ny, int nz, const double* in, double* out, bool forward)
 {
     int rc = check_dims(nx, ny, nz);
     if (rc) return rc;
     const int nh = nx / 2 + 1, ld = spec_ld(nx);
     const size_t rows = (size_t)ny * nz, drows = rows < 8 ? 8 : rows;
-    DevBuf spec, real;
-    if ((rc = spec.alloc(drows * ld * sizeof(double2)))) return rc;
-    if ((rc = real.alloc(drows * nx * sizeof(double)))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto& ws = g_ws[current_device()];
+    if (!ws) ws = std::make_unique<HostWs>();
+    DevBuf& spec = ws->spec;
+    DevBuf& real = ws->real;
+    if (spec.bytes < drows * ld * sizeof(double2) && (rc = spec.alloc(drows * ld * sizeof(double2)))) return rc;
+    if (real.bytes < drows * nx * sizeof(double) && (rc = real.alloc(drows * nx * sizeof(double)))) return rc;
     hipStream_t st = nullptr;
     double2* S = spec.as<double2>();
     if (forward) {
-        if (drows > rows) TARA_HIP(hipMemset(real.p, 0, real.bytes));
+        if (drows > rows) TARA_HIP(hipMemset(real.p, 0, drows * nx * sizeof(double)));
         TARA_HIP(hipMemcpy(real.p, in, rows * nx * sizeof(double), hipMemcpyHostToDevice));
         if ((rc = row_r2c(real.as<double>(), S, nx, ld, (int)drows, st))) return rc;
         if (ny > 1 && (rc = col_fft(S, ny, ld, ld, (long)ld * ny, nz, -1, st))) return rc;
