@@ -263,8 +263,9 @@ __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
 #pragma unroll
     for (int e = 0; e < E; ++e) nl[e] = (nl[e] + v[e].x * v[e].y) * scale;
     // forward x transform of the real product row: pack z[m] = nl[2m] + i nl[2m+1]
-    // for the half-length FFT (threads t < TM carry it; the others run the same
-    // code on the unused upper half of the row buffer so every barrier matches).
+    // for the half-length FFT (threads t < TM carry it; the others only meet its
+    // barriers where they fill whole waves, else run the same code on the
+    // unused upper half of the row buffer so every barrier matches).
     double* xr = reinterpret_cast<double*>(xl);
 #pragma unroll
     for (int e = 0; e < E; ++e) xr[rf_final_idx<NX, EM>(t, e)] = nl[e];
@@ -285,9 +286,18 @@ __global__ __launch_bounds__((XShape<NX, EM, RB, SPLIT>::NT)) void k_csh_x(
 #pragma unroll
     for (int u = 0; u < NK; ++u) wk[u] = rctw[t + u * T];
     __syncthreads();
-    constexpr int HALF = SPLIT ? RF<M, EM>::LDS / 2 : RF<M, EM>::LDS;  // upper half, complex elements
-    double2* zl = xl + (t < S::TM ? 0 : HALF);
-    rf_fft<M, -1, EM, false, SPLIT>(z, tm, zl, tw_m);
+    if constexpr ((S::TM * RB) % 64 == 0) {
+        // the threads t >= TM (threadIdx.x >= TM RB) fill whole waves: they meet
+        // the transform's barriers without running it
+        if (t < S::TM)
+            rf_fft<M, -1, EM, false, SPLIT>(z, tm, xl, tw_m);
+        else
+            rf_fft_barriers_only<M, EM, SPLIT>();
+    } else {
+        constexpr int HALF = SPLIT ? RF<M, EM>::LDS / 2 : RF<M, EM>::LDS;  // upper half, complex elements
+        double2* zl = xl + (t < S::TM ? 0 : HALF);
+        rf_fft<M, -1, EM, false, SPLIT>(z, tm, zl, tw_m);
+    }
     if (t < S::TM) {
 #pragma unroll
         for (int e = 0; e < EH; ++e) xl[rf_pad(rf_final_idx<M, EM>(tm, e))] = z[e];

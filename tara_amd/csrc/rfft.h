@@ -214,6 +214,28 @@ __device__ __forceinline__ void rf_fft(double2 (&v)[(RF<N, EM>::E)], int t, doub
     rf_rec<N, 1, DIR, EM, WAVE, SPLIT, TWPRE>(v, t, lds, tw, first);
 }
 
+// Number of LDS exchanges of a length-n transform with radix cap e: one
+// between consecutive passes of the schedule rf_rec walks (span L, radix
+// fft_radix(n / L, e), exchange while L * radix < n).
+__host__ __device__ constexpr int rf_exchanges(int n, int e)
+{
+    int x = 0;
+    for (int span = 1; span * fft_radix(n / span, e) < n; span *= fft_radix(n / span, e)) ++x;
+    return x;
+}
+
+// The workgroup barriers of rf_fft<N, DIR, EM, false, SPLIT> without its
+// work: for the waves of a workgroup that hold no sequence of this transform,
+// so that every wave meets the same barriers (2 per exchange, 4 when SPLIT).
+// Callers must branch on a wave-uniform condition.
+template <int N, int EM = 8, bool SPLIT = false>
+__device__ __forceinline__ void rf_fft_barriers_only()
+{
+    constexpr int nb = rf_exchanges(N, RF<N, EM>::E) * (SPLIT ? 4 : 2);
+#pragma unroll
+    for (int b = 0; b < nb; ++b) __syncthreads();
+}
+
 // Reorders registers from final (output) positions to first-pass (input)
 // positions through LDS, e.g. to feed a transform's output into another
 // transform of the same length.
