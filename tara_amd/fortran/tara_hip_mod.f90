@@ -141,6 +141,112 @@ module tara_hip_mod
       import :: c_ptr
     end function
 
+    ! Device-resident mode (include/tara_hip.h): the state stays on the GPU
+    ! between steps; handles are opaque c_ptr values.
+    integer(c_int) function tara_csh_create(nx, ny, nu, dt, scheme, ctx) bind(C, name="tara_csh_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nx, ny, scheme
+      real(c_double), value :: nu, dt
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function tara_csh_upload(ctx, omegak) bind(C, name="tara_csh_upload")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: omegak(*)
+    end function
+
+    integer(c_int) function tara_csh_download(ctx, omegak) bind(C, name="tara_csh_download")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: omegak(*)
+    end function
+
+    integer(c_int) function tara_csh_advance(ctx, nsteps) bind(C, name="tara_csh_advance")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function tara_csh_sync(ctx) bind(C, name="tara_csh_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function tara_csh_destroy(ctx) bind(C, name="tara_csh_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function tara_csh_ic_shear(ctx, lx_in, ly_in, w0, m0, d_in) bind(C, name="tara_csh_ic_shear")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: lx_in, ly_in, w0, m0, d_in
+    end function
+
+    integer(c_int) function tara_csh_fields(ctx, omega, ux, uy) bind(C, name="tara_csh_fields")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: omega(*), ux(*), uy(*)
+    end function
+
+    integer(c_int) function tara_csh_advance_tracers(ctx, nsteps, dt, lx, ly, periodic, np, xp, yp) &
+        bind(C, name="tara_csh_advance_tracers")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps, periodic, np
+      real(c_double), value :: dt, lx, ly
+      real(c_double), intent(inout) :: xp(*), yp(*)
+    end function
+
+    integer(c_int) function tara_mhd_create(nx, ny, nz, p, ctx) bind(C, name="tara_mhd_create")
+      import :: c_int, c_ptr, tara_mhd_params
+      integer(c_int), value :: nx, ny, nz
+      type(tara_mhd_params), intent(in) :: p
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function tara_mhd_upload(ctx, state) bind(C, name="tara_mhd_upload")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(in) :: state(*)
+    end function
+
+    integer(c_int) function tara_mhd_download(ctx, state) bind(C, name="tara_mhd_download")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(in) :: state(*)
+    end function
+
+    integer(c_int) function tara_mhd_advance(ctx, nsteps) bind(C, name="tara_mhd_advance")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function tara_mhd_sync(ctx) bind(C, name="tara_mhd_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function tara_mhd_destroy(ctx) bind(C, name="tara_mhd_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function tara_mhd_ic_abc(ctx, rho0, u0, b0, p0, a, b, cc, kf) bind(C, name="tara_mhd_ic_abc")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: rho0, u0, b0, p0, a, b, cc, kf
+    end function
+
+    integer(c_int) function tara_mhd_record_energy(ctx, nrec, every, rows) bind(C, name="tara_mhd_record_energy")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nrec, every
+      real(c_double), intent(out) :: rows(*)
+    end function
+
     ! checkpoints (include/tara_hip.h; host I/O)
     integer(c_int) function tara_ckpt_write(path, nx, ny, nz, nfields, step, time, fields) &
         bind(C, name="tara_ckpt_write")

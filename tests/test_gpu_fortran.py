@@ -167,3 +167,44 @@ def test_dfftw_shim_vs_oracle(tmp_path):
     assert rel_l2(b3, 1024 * r3) < 1e-14
     assert rel_l2(c1, np.fft.rfft(r1)) < 1e-14
     assert rel_l2(b1, 32 * r1) < 1e-14
+
+
+@pytest.mark.gpu
+def test_fortran_resident_mode(tmp_path):
+    """The device-resident API from Fortran (module tara_hip_mod,
+    tests/fortran/drive_resident.f90): the config-1 run with tracer particles
+    kept on the GPU for 1001 steps against the reference program's snapshots
+    (tests/golden/golden_snap.npz; particles that never enter the last row),
+    and 20 rows of the 3D Energy.dat averages against the same calls made
+    through the Python binding (the same library: equal bits)."""
+    import tara_amd as T
+
+    g = np.load(os.path.join(ROOT, "tests", "golden", "golden_snap.npz"))
+    start = g["s64_start"]
+    npart = start.shape[0]
+    with open(tmp_path / "in.bin", "wb") as f:
+        np.array([npart], dtype="<i4").tofile(f)
+        start[:, 0].astype("<f8").tofile(f)
+        start[:, 1].astype("<f8").tofile(f)
+    print("fortran resident", flush=True)
+    subprocess.run([_driver("drive_resident"), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], check=True,
+                   timeout=300)
+    out = np.fromfile(tmp_path / "out.bin", dtype="<f8")
+    n2, nrec = 64, 20
+    xp, yp = out[:npart], out[npart:2 * npart]
+    fields = out[2 * npart:2 * npart + 3 * n2 * n2].reshape(3, n2, n2)  # each (x, y) column-major: [y][x]
+    rows = out[2 * npart + 3 * n2 * n2:].reshape(nrec, 10)
+    ok = ~g["s64_last_row"]
+    p = g["s64_particles_1000"]
+    assert np.max(np.abs(xp[ok] - p[ok, 0])) < 1e-10 and np.max(np.abs(yp[ok] - p[ok, 1])) < 1e-10
+    f = g["s64_fields_1000"]
+    for c in range(3):
+        got = fields[c].T  # (x, y)
+        ref = f[:, 2 + c].reshape(n2, n2)
+        assert np.max(np.abs(got - ref)) < 1e-10 * np.max(np.abs(ref)), c
+    L = 2.0 * np.pi
+    s = T.MhdSolver(16, 16, 16, T.MhdParams(L, L, L, 1.0 / 450.0, 1.0 / 450.0, 10.0, 1e-3))
+    s.ic_abc(1.0, 0.1, 0.1, 1.0, 1.0, 1.0, 1.0, 1.0)
+    want = s.record_energy(nrec, 1)
+    s.close()
+    assert np.array_equal(rows, np.asarray(want).reshape(nrec, 10))
