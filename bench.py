@@ -238,7 +238,9 @@ def bench_ensemble(steps, warmup, nruns=512, seed=7):
             "sequential_value": pts / seq, "sequential_ms_per_step": seq * 1e3,
             "single_run_ms_per_step": {n: round(v * 1e3, 4) for n, v in singles.items()},
             "speedup_vs_sequential": seq / t_ens,
-            "step_model_bytes": 60 * S_all, "step_model_GBps": round(60 * S_all / t_ens / 1e9, 1)}
+            "step_model_bytes": 60 * S_all, "step_model_GBps": round(60 * S_all / t_ens / 1e9, 1),
+            # FP64 transform work: 4 RHS x 5 real 2D transforms of 2.5 N log2 N flops per run
+            "fp64_fft_tflops": round(sum(20 * 2.5 * n * n * math.log2(n * n) for n in sizes) / t_ens / 1e12, 2)}
 
 
 def fno_forward_flops(n, c, mx, my, layers):
