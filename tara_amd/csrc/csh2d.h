@@ -65,6 +65,8 @@ struct CshPlan {
     const double2* tw_xn = nullptr;  // length nx (paired backward x transforms)
     const double2* tw_x16 = nullptr;   // the same two tables for radix-16 schedules
     const double2* tw_xn16 = nullptr;
+    const double2* tw_x4 = nullptr;    // and for radix-4 schedules
+    const double2* tw_xn4 = nullptr;
     int xcfg = 0;                    // x-pass shape (see csh_xshape_ok / the autotuner)
     const double2* rctw = nullptr;
     DevBuf bad;                      // int per run: set when a step's new state holds a non-finite value

@@ -219,8 +219,8 @@ __device__ __forceinline__ void x_c2r_pair(const double2* ga, const double2* gb,
     rf_fft<NX, +1, EM, false, SPLIT>(v, t, xl, tw);
 }
 
-// Shape of the x pass: EM complex elements per thread (8: radix-8 passes; 16:
-// radix-16), RB rows per workgroup, SPLIT: the FFT exchanges go through LDS as
+// Shape of the x pass: EM complex elements per thread (4, 8, 16: radix-4, -8,
+// -16 passes), RB rows per workgroup, SPLIT: the FFT exchanges go through LDS as
 // doubles in two rounds (half the LDS, so that two workgroups share a CU).
 template <int NX, int EM, int RB, bool SPLIT>
 struct XShape {
@@ -533,7 +533,8 @@ static int launch_yinv(const CshPlan& p, const double2* w, const CshPhys& ph, hi
 struct XCand {
     int em, rb, split;
 };
-constexpr XCand kXCands[] = {{8, 0, 0}, {16, 2, 1}, {8, 1, 1}, {16, 4, 1}, {8, 2, 1}, {16, 1, 0}};
+constexpr XCand kXCands[] = {{8, 0, 0}, {16, 2, 1}, {8, 1, 1}, {16, 4, 1}, {8, 2, 1}, {16, 1, 0},
+                              {4, 1, 0}, {4, 2, 0}, {4, 4, 1}};
 constexpr int kNumXCands = sizeof(kXCands) / sizeof(kXCands[0]);
 
 template <int NX, int EM>
@@ -552,8 +553,9 @@ static int launch_x_shape(const CshPlan& p, hipStream_t st)
     } else {
         TARA_CHECK(p.ny % RB == 0, TARA_ERR_UNSUPPORTED, "Ny=%d not a multiple of %d", p.ny, RB);
         const double scale = 1.0 / ((double)p.nx * (double)p.ny * (double)p.nx * (double)p.ny);
-        const double2* twn = EM == 16 ? p.tw_xn16 : p.tw_xn;
-        const double2* twm = EM == 16 ? p.tw_x16 : p.tw_x;
+        const double2* twn = EM == 16 ? p.tw_xn16 : EM == 4 ? p.tw_xn4 : p.tw_xn;
+        const double2* twm = EM == 16 ? p.tw_x16 : EM == 4 ? p.tw_x4 : p.tw_x;
+        TARA_CHECK(twn && twm, TARA_ERR_UNSUPPORTED, "x pass shape without its twiddle tables");
         hipLaunchKernelGGL((k_csh_x<NX, EM, RB, SPLIT>), dim3(p.ny / RB, p.batch), dim3(S::NT), 0, st,
                            p.g[0].as<double2>(), p.g[1].as<double2>(), p.g[2].as<double2>(), p.g[3].as<double2>(),
                            scale, p.h.as<double2>(), twn, twm, p.rctw, p.bstride);
@@ -571,6 +573,9 @@ static int launch_x(const CshPlan& p, hipStream_t st)
         case 3: return launch_x_shape<NX, 16, 4, true>(p, st);
         case 4: return launch_x_shape<NX, 8, 2, true>(p, st);
         case 5: return launch_x_shape<NX, 16, 1, false>(p, st);
+        case 6: return launch_x_shape<NX, 4, 1, false>(p, st);
+        case 7: return launch_x_shape<NX, 4, 2, false>(p, st);
+        case 8: return launch_x_shape<NX, 4, 4, true>(p, st);
         default: set_error("x pass shape %d unknown", p.xcfg); return TARA_ERR_ARG;
     }
 }
@@ -585,6 +590,9 @@ static bool x_shape_ok(int c, int ny)
         case 3: return XShape<NX, 16, 4, true>::OK && ny % 4 == 0;
         case 4: return XShape<NX, 8, 2, true>::OK && ny % 2 == 0;
         case 5: return XShape<NX, 16, 1, false>::OK;
+        case 6: return XShape<NX, 4, 1, false>::OK;
+        case 7: return XShape<NX, 4, 2, false>::OK && ny % 2 == 0;
+        case 8: return XShape<NX, 4, 4, true>::OK && ny % 4 == 0;
         default: return false;
     }
 }
@@ -732,8 +740,10 @@ int csh_plan_init(CshPlan& p, int nx, int ny, int batch)
     p.tw_xn = twiddles(nx);
     p.tw_x16 = twiddles(nx / 2, 16);
     p.tw_xn16 = twiddles(nx, 16);
+    p.tw_x4 = twiddles(nx / 2, 4);
+    p.tw_xn4 = twiddles(nx, 4);
     p.rctw = rc_twiddles(nx);
-    TARA_CHECK(p.tw_y && p.tw_x && p.tw_xn && p.tw_x16 && p.tw_xn16 && p.rctw, TARA_ERR_ALLOC,
+    TARA_CHECK(p.tw_y && p.tw_x && p.tw_xn && p.tw_x16 && p.tw_xn16 && p.tw_x4 && p.tw_xn4 && p.rctw, TARA_ERR_ALLOC,
                "twiddle table allocation failed");
     p.xcfg = csh_default_xshape(nx);
     if (shape_ok(p, p.xcfg) != 1) p.xcfg = 0;
