@@ -207,7 +207,7 @@ def test_advance_one_call_matches_single_steps(n, scheme):
 
 
 # ------------------------------------------------------------------ x-pass shapes
-@pytest.mark.parametrize("nx,ny", [(64, 64), (256, 512), (1024, 1024), (4096, 256)])
+@pytest.mark.parametrize("nx,ny", [(64, 64), (256, 512), (1024, 1024), (4096, 256), (4096, 1024), (2048, 2048)])
 def test_every_xshape_matches_oracle(nx, ny):
     """Every x-pass shape the autotuner may choose gives the same AB2 step
     (w + 3/2 dt f from zero history) as the oracle."""
