@@ -27,17 +27,22 @@ struct EventTimer {
     ~EventTimer();
 };
 
+// Elements per thread (also the radix cap) of the y transforms for column
+// length ny: 4 at ny = 1024, where a single run's 513 columns at 8 elements
+// per thread would fill one wave per SIMD; 8 otherwise.
+__host__ __device__ constexpr int csh_yem(int ny) { return ny == 1024 ? 4 : 8; }
+
 // Shape of the inverse y pass (k_csh_yinv) for column length ny: whether two
 // columns share a workgroup interleaved across its lanes (ny = 4096, 512
 // threads per column), and how many columns a workgroup holds (otherwise
-// contiguous, ny/8 threads each, 256 threads at least).
+// contiguous, ny/csh_yem(ny) threads each, 256 threads at least).
 __host__ __device__ constexpr bool csh_yinv_interleaved(int ny, bool screened)
 {
     return !screened && ny == 4096;
 }
 __host__ __device__ constexpr int csh_yinv_cols(int ny, bool screened)
 {
-    const int t = ny / (ny < 8 ? ny : 8);
+    const int t = ny / (ny < csh_yem(ny) ? ny : csh_yem(ny));
     return csh_yinv_interleaved(ny, screened) ? 2 : (t >= 256 ? 1 : 256 / t);
 }
 // Layout of the intermediates G0..G3 (see blkg), a function of nh = nx/2 + 1

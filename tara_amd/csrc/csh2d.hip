@@ -36,10 +36,14 @@
 
 namespace tara {
 
+// The register-resident transform of the y passes (csh_yem elements per thread).
+template <int NY>
+using RY = RF<NY, csh_yem(NY)>;
+
 // Columns per workgroup of the y passes: at least 256 threads.
 template <int N>
 struct YShape {
-    static constexpr int T = RF<N>::T;
+    static constexpr int T = RY<N>::T;
     static constexpr int CB = T >= 256 ? 1 : 256 / T;
     static constexpr int NT = CB * T;
 };
@@ -84,12 +88,12 @@ __device__ __forceinline__ double csh_ky(int j)
 // registers), keep the contiguous shape of the y passes.
 template <int NY, bool SCR>
 struct YiShape {
-    static constexpr int T = RF<NY>::T;
+    static constexpr int T = RY<NY>::T;
     static constexpr bool IL = csh_yinv_interleaved(NY, SCR);
     static constexpr int CB = csh_yinv_cols(NY, SCR);
     static_assert(IL || CB == YShape<NY>::CB, "contiguous shape differs from the y passes'");
     static constexpr int NT = CB * T;
-    static constexpr int LDSC = RF<NY>::LDS + (IL ? 4 : 0);  // per-column exchange buffer
+    static constexpr int LDSC = RY<NY>::LDS + (IL ? 4 : 0);  // per-column exchange buffer
     // Ny <= 512: at least 56 KB of LDS per workgroup, which holds the pass to
     // two 256-thread workgroups per CU.  Its registers would allow three, and
     // at three the ensemble step (128-512 grids) measured 6.67 against 6.42 ms.
@@ -102,12 +106,12 @@ struct YiShape {
 // derive2css.f95:22-52 when SCR (screened), runs the backward y FFTs and writes
 // G0..G3 (row-group stride ld).  x is the column's LDS exchange buffer.
 template <int NY, bool SCR, int GS, bool OPQ>
-__device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i, bool live, int t, double2* x,
+__device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RY<NY>::E], int i, bool live, int t, double2* x,
                                                 const double2* tw, double lam, int ld,
                                                 double2* __restrict__ g0, double2* __restrict__ g1,
                                                 double2* __restrict__ g2, double2* __restrict__ g3)
 {
-    constexpr int E = RF<NY>::E;
+    constexpr int E = RY<NY>::E;
     const double kxv = (double)i;  // derive2csh's kx = i-1 exactly (checked on the host)
     // screened: psi = omega/(k^2 + lambda) with psi(0,0) = 0, omega_d = k^2 psi;
     // the derivatives below act on omega_d, psi is kept in ps.
@@ -115,7 +119,7 @@ __device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i,
     if constexpr (SCR) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const int j = rf_first_idx<NY>(t, e);
+            const int j = rf_first_idx<NY, csh_yem(NY)>(t, e);
             const double kyv = csh_ky<NY>(j);
             const double k2 = kxv * kxv + kyv * kyv;
             const double den = k2 + lam;
@@ -138,7 +142,7 @@ __device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i,
                 if constexpr (SCR) {
                     wv[e] = ps[e];
                 } else {
-                    const double kyv = csh_ky<NY>(rf_first_idx<NY>(tt, e));
+                    const double kyv = csh_ky<NY>(rf_first_idx<NY, csh_yem(NY)>(tt, e));
                     const double k2 = kxv * kxv + kyv * kyv;
                     wv[e] = (k2 != 0.0) ? make_double2(wv[e].x / k2, wv[e].y / k2) : make_double2(0.0, 0.0);
                 }
@@ -146,18 +150,18 @@ __device__ __forceinline__ void csh_yinv_column(double2 (&wv)[RF<NY>::E], int i,
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const double kyv = csh_ky<NY>(rf_first_idx<NY>(tt, e));
+            const double kyv = csh_ky<NY>(rf_first_idx<NY, csh_yem(NY)>(tt, e));
             const double kk = (f == 0 || f == 3) ? kxv : kyv;
             const double2 z = wv[e];
             v[e] = (f == 3) ? make_double2(kk * z.y, -(kk * z.x))     // -i kx psi
                             : make_double2(-(kk * z.y), kk * z.x);    // i k (omega | psi)
         }
-        rf_fft<NY, +1>(v, tt, x, tw);
+        rf_fft<NY, +1, csh_yem(NY)>(v, tt, x, tw);
         // G0 = ux (f 2), G1 = uy (f 3), G2 = d omega/dx (f 0), G3 = d omega/dy (f 1)
         double2* o = f == 0 ? g2 : f == 1 ? g3 : f == 2 ? g0 : g1;
         if (live) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) o[blkg(i, rf_final_idx<NY>(tt, e), ld, GS)] = v[e];
+            for (int e = 0; e < E; ++e) o[blkg(i, rf_final_idx<NY, csh_yem(NY)>(tt, e), ld, GS)] = v[e];
         }
     }
 }
@@ -176,7 +180,7 @@ __global__ __launch_bounds__((YiShape<NY, SCR>::NT), (YiShape<NY, SCR>::IL ? 4 :
     g2 += rb;
     g3 += rb;
     using S = YiShape<NY, SCR>;
-    constexpr int E = RF<NY>::E, T = S::T;
+    constexpr int E = RY<NY>::E, T = S::T;
     __shared__ double2 lds[S::LDS_ELEMS];
     const int c = S::IL ? (int)threadIdx.x % S::CB : (int)threadIdx.x / T;
     const int t = S::IL ? (int)threadIdx.x / S::CB : (int)threadIdx.x % T;
@@ -185,7 +189,7 @@ __global__ __launch_bounds__((YiShape<NY, SCR>::NT), (YiShape<NY, SCR>::IL ? 4 :
     const size_t col = (size_t)(live ? i : 0) * NY;
     double2 wv[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) wv[e] = live ? w[col + rf_first_idx<NY>(t, e)] : make_double2(0.0, 0.0);
+    for (int e = 0; e < E; ++e) wv[e] = live ? w[col + rf_first_idx<NY, csh_yem(NY)>(t, e)] : make_double2(0.0, 0.0);
     csh_yinv_column<NY, SCR, GS, S::IL>(wv, i, live, t, lds + c * S::LDSC, tw, lam, ld, g0, g1, g2, g3);
 }
 
@@ -344,22 +348,22 @@ __global__ __launch_bounds__(YShape<NY>::NT) void k_csh_yfwd(YfwdArgs a, const d
     if (a.next) a.next += rb;
     if (a.nus) a.nu = a.nus[blockIdx.y];
     using S = YShape<NY>;
-    constexpr int E = RF<NY>::E, T = S::T;
-    __shared__ double2 lds[S::CB * RF<NY>::LDS];
+    constexpr int E = RY<NY>::E, T = S::T;
+    __shared__ double2 lds[S::CB * RY<NY>::LDS];
     const int c = threadIdx.x / T, t = threadIdx.x % T;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * S::CB + c;
     const bool live = i < a.nh;
     const size_t col = (size_t)(live ? i : 0) * NY;
     double2 v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = live ? a.h[blk8(i, rf_first_idx<NY>(t, e), a.nh)] : make_double2(0.0, 0.0);
-    rf_fft<NY, -1>(v, t, lds + c * RF<NY>::LDS, tw);
+    for (int e = 0; e < E; ++e) v[e] = live ? a.h[blk8(i, rf_first_idx<NY, csh_yem(NY)>(t, e), a.nh)] : make_double2(0.0, 0.0);
+    rf_fft<NY, -1, csh_yem(NY)>(v, t, lds + c * RY<NY>::LDS, tw);
     if (!live) return;
     const double kxv = (double)i;
     bool finite = true;  // the new state's values (AB2, RK4 stage 4)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int j = rf_final_idx<NY>(t, e);
+        const int j = rf_final_idx<NY, csh_yem(NY)>(t, e);
         const size_t q = col + j;
         double2 nl = v[e];
         if (i >= a.imin[j]) nl = make_double2(0.0, 0.0);
@@ -436,7 +440,7 @@ __global__ __launch_bounds__((YiShape<NY, false>::NT), (YiShape<NY, false>::IL ?
     g3 += rb;
     if (a.nus) a.nu = a.nus[blockIdx.y];
     using S = YiShape<NY, false>;
-    constexpr int E = RF<NY>::E, T = S::T;
+    constexpr int E = RY<NY>::E, T = S::T;
     __shared__ double2 lds[S::LDS_ELEMS];
     const int c = S::IL ? (int)threadIdx.x % S::CB : (int)threadIdx.x / T;
     const int t = S::IL ? (int)threadIdx.x / S::CB : (int)threadIdx.x % T;
@@ -446,13 +450,13 @@ __global__ __launch_bounds__((YiShape<NY, false>::NT), (YiShape<NY, false>::IL ?
     double2* xl = lds + c * S::LDSC;
     double2 v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = live ? a.h[blk8(i, rf_first_idx<NY>(t, e), a.nh)] : make_double2(0.0, 0.0);
-    rf_fft<NY, -1>(v, t, xl, tw);
+    for (int e = 0; e < E; ++e) v[e] = live ? a.h[blk8(i, rf_first_idx<NY, csh_yem(NY)>(t, e), a.nh)] : make_double2(0.0, 0.0);
+    rf_fft<NY, -1, csh_yem(NY)>(v, t, xl, tw);
     const double kxv = (double)i;
     bool finite = true;  // the new state's values (AB2, RK4 stage 4)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int j = rf_final_idx<NY>(t, e);
+        const int j = rf_final_idx<NY, csh_yem(NY)>(t, e);
         const size_t q = col + j;
         double2 nl = v[e];
         if (!live || i >= a.imin[j]) nl = make_double2(0.0, 0.0);
@@ -495,7 +499,7 @@ __global__ __launch_bounds__((YiShape<NY, false>::NT), (YiShape<NY, false>::IL ?
         v[e] = wn;
     }
     if (!finite && a.bad) a.bad[blockIdx.y] = 1;
-    if constexpr (!rf_final_is_first<NY, 8>()) rf_final_to_first<NY>(v, t, xl);
+    if constexpr (!rf_final_is_first<NY, csh_yem(NY)>()) rf_final_to_first<NY, csh_yem(NY)>(v, t, xl);
     csh_yinv_column<NY, false, GS, S::IL>(v, i, live, t, xl, tw, 0.0, ld, g0, g1, g2, g3);
 }
 
@@ -735,7 +739,7 @@ int csh_plan_init(CshPlan& p, int nx, int ny, int batch)
     TARA_HIP(hipMemcpy(p.kx.p, kx.data(), p.nh * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.ky.p, ky.data(), ny * sizeof(double), hipMemcpyHostToDevice));
     TARA_HIP(hipMemcpy(p.imin.p, imin.data(), ny * sizeof(int), hipMemcpyHostToDevice));
-    p.tw_y = twiddles(ny);
+    p.tw_y = twiddles(ny, csh_yem(ny));
     p.tw_x = twiddles(nx / 2);
     p.tw_xn = twiddles(nx);
     p.tw_x16 = twiddles(nx / 2, 16);

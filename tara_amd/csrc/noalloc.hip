@@ -53,7 +53,7 @@ Layout layout_of(int nx, int ny)
     L.kx = take(nh * sizeof(double));
     L.ky = take(ny * sizeof(double));
     L.imin = take(ny * sizeof(int));
-    L.tw_y = take(std::max(1, fft_tw_size(ny, 8)) * sizeof(double2));
+    L.tw_y = take(std::max(1, fft_tw_size(ny, csh_yem(ny))) * sizeof(double2));
     L.tw_x = take(std::max(1, fft_tw_size(nx / 2, 8)) * sizeof(double2));
     L.tw_xn = take(std::max(1, fft_tw_size(nx, 8)) * sizeof(double2));
     L.tw_x16 = take(std::max(1, fft_tw_size(nx / 2, 16)) * sizeof(double2));
@@ -100,7 +100,7 @@ int put_tables(int dev, char* base, const Layout& L, int nx, int ny, hipStream_t
     const struct {
         size_t off;
         int n, rmax;
-    } tws[] = {{L.tw_y, ny, 8}, {L.tw_x, nx / 2, 8}, {L.tw_xn, nx, 8}, {L.tw_x16, nx / 2, 16}, {L.tw_xn16, nx, 16}};
+    } tws[] = {{L.tw_y, ny, csh_yem(ny)}, {L.tw_x, nx / 2, 8}, {L.tw_xn, nx, 8}, {L.tw_x16, nx / 2, 16}, {L.tw_xn16, nx, 16}};
     for (const auto& w : tws) {
         host_twiddles(w.n, w.rmax, t);
         TARA_HIP(hipMemcpy(base + w.off, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
