@@ -5,12 +5,13 @@
 // Device layout: the state half spectra are stored ky-major ("transposed"),
 // element (kx = i, ky = j) at i*Ny + j, so a kx column (one y transform) is a
 // contiguous Ny-element run.  The intermediates are stored in row groups
-// interleaved per kx: G0..G3 in 4-row groups (blkg), H in 8-row groups (blk8,
-// element (i, j) at ((j/8)*Nh + i)*8 + j%8): a y pass reads or writes a column
-// as 64- or 128-byte runs, and the x pass (RB = 2 rows per workgroup at
-// Nx = 4096) reads 32-byte pieces of those runs, whose other pieces belong to the
-// neighbouring workgroups that the XCD-aware mapping runs next to it.  The reference's (Nh, Ny) kx-fastest arrays
-// are converted on upload / download (layout.hip).
+// interleaved per kx: G0..G3 in 2-row (Nx = 4096) or 4-row groups (blkg), H in
+// 8-row groups (blk8, element (i, j) at ((j/8)*Nh + i)*8 + j%8): a y pass reads
+// or writes a column as 32- to 128-byte runs, and an x-pass workgroup of fewer
+// rows than a group reads pieces of those runs whose other pieces belong to the
+// neighbouring workgroups, which the XCD-aware mapping runs next to it.  The
+// reference's (Nh, Ny) kx-fastest arrays are converted on upload / download
+// (layout.hip).
 //
 //   pass 1  k_csh_yinv : per kx column read omega_hat, form the four spectra of
 //                        derive2csh.f95:29-71 (i kx omega, i ky omega, i ky psi,
@@ -47,8 +48,6 @@ struct YShape {
     static constexpr int CB = T >= 256 ? 1 : 256 / T;
     static constexpr int NT = CB * T;
 };
-// Rows per workgroup of the x pass: 512 threads where that fits (RB = 4 rows,
-// 64 contiguous bytes per kx, at Nx = 4096), at most 8 rows.
 // Offset of element (kx = i, ky = j) in an 8-row-blocked intermediate.
 __device__ __forceinline__ size_t blk8(int i, int j, int nh)
 {
