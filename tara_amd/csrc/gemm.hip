@@ -137,6 +137,21 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g, int avec, int bvec)
     v16f acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    // result register i of lane l: row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32
+    const long gn = n0 + wn + lane % 32;
+    // the epilogue's reads of C, issued with the first operand tiles so that
+    // their latency overlaps the products instead of following them
+    constexpr bool READC = EPI == GEMM_ACCUM_F32 || EPI == GEMM_ADD_TO;
+    float cin[READC ? 16 : 1];
+    if constexpr (READC) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const long gm = m0 + wm + 8 * (i / 4) + 4 * (lane / 32) + (i % 4);
+            cin[i] = (gm < g.m && gn < g.n)
+                         ? reinterpret_cast<const float*>(g.c)[z1 * g.sc1 + z2 * g.sc2 + gm * g.scm + gn * g.scn]
+                         : 0.f;
+        }
+    }
     T ra[16], rb[16];
     // B is fetched as rows n, columns k: element (n, k) at B[n*sbn + k*sbk]
     fetch_tile<T, AM>(A, g.sam, g.sak, m0, 0, g.m, g.k, avec != 0, ra);
@@ -170,8 +185,6 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g, int avec, int bvec)
         }
         __syncthreads();
     }
-    // result register i of lane l: row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32
-    const long gn = n0 + wn + lane % 32;
     if (gn >= g.n) return;
     const bool extra = g.bias || g.act || g.pre || g.c_f32 || g.c_bf16;
 #pragma unroll
@@ -180,7 +193,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g, int avec, int bvec)
         if (gm >= g.m) continue;
         const long off = z1 * g.sc1 + z2 * g.sc2 + gm * g.scm + gn * g.scn;
         float v = acc[i] * g.alpha;
-        if constexpr (EPI == GEMM_ACCUM_F32 || EPI == GEMM_ADD_TO) v += reinterpret_cast<const float*>(g.c)[off];
+        if constexpr (READC) v += cin[i];
         if (extra) {
             if (g.bias) v += g.bias[gm];
             if (g.pre) g.pre[off] = v;
