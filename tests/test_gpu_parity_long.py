@@ -12,7 +12,10 @@
   * one derive2csh call and one resident RK4 step at 4096^2 (the benchmark
     layout: interleaved inverse y columns, 2-row groups, padded rows) on a
     full-spectrum field with non-Hermitian kx = 0 and Nyquist columns,
-    against the reference computed on the host.
+    against the reference computed on the host;
+  * one derive3com call and one resident AB2 step at 256^3 (the benchmark
+    size; padded rows, every mode non-zero, the kx = 0 and Nyquist planes not
+    Hermitian) against the reference's loop on the host.
 
 Bar: relative L2 <= 1e-10 after the stated step counts (the FP64 parity
 target); single calls 1e-12.
@@ -26,6 +29,7 @@ import oracle as O
 import tara_amd as T
 from conftest import ROOT, rel_l2
 from ref_child import run as ref_run
+from ref_child import run_mhd3
 from test_oracle_pins import LIVE, shear_rhs, shear_state
 
 pytestmark = pytest.mark.gpu
@@ -159,3 +163,45 @@ def test_4096_full_spectrum_derive_and_rk4_step_vs_reference():
     s.close()
     want = ref_run("rk4", w, n, nu, dt, 1)
     assert rel_l2(got, want) < 1e-12
+
+
+def _full_spectrum_mhd(n, seed=7):
+    """rho = 1 plus small fluctuations, velocity and magnetic fields of O(0.5):
+    every mode non-zero with amplitude ~ exp(-|k|/24), so the Nyquist planes
+    and the kx = 0 plane (which the c2r reads only through its Hermitian
+    part) carry data."""
+    rng = np.random.default_rng(seed)
+    nh = n // 2 + 1
+    kx = np.arange(nh, dtype=float)[:, None, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :, None]
+    kz = np.fft.fftfreq(n, 1.0 / n)[None, None, :]
+    amp = float(n) ** 3 * np.exp(-np.sqrt(kx * kx + ky * ky + kz * kz) / 24.0)
+    del kx, ky, kz
+
+    def field(a):
+        return np.asfortranarray(a * amp * (rng.standard_normal(amp.shape) + 1j * rng.standard_normal(amp.shape)))
+
+    rho = field(1e-4)
+    rho[0, 0, 0] = float(n) ** 3
+    st = [rho] + [field(2.4e-3) for _ in range(3)] + [field(1e-3)] + [field(1e-3) for _ in range(3)]
+    return st
+
+
+def test_256_full_spectrum_derive_and_ab2_step_vs_reference():
+    if not O.ref_available():
+        pytest.skip("the reference build (oracle/_ref) is not present")
+    n, mu, eta, dt = 256, 1e-3, 1e-3, 5e-4
+    st = _full_spectrum_mhd(n)
+    print("256^3 derive3com (drop-in) and one resident AB2 step", flush=True)
+    f_gpu = T.derive3com(n, n, n, n // 2 + 1, params(mu, eta, dt), st)
+    s = T.MhdSolver(n, n, n, params(mu, eta, dt))
+    s.upload(st)
+    s.advance(1)
+    got = s.download()
+    s.close()
+    print("256^3 reference derive3com + ab3com on the host", flush=True)
+    want, f_ref = run_mhd3(st, n, mu, eta, dt, 1)
+    for k in LIVE:
+        assert rel_l2(f_gpu[k], f_ref[k]) < 1e-12, ("rhs", k)
+    for k in range(8):
+        assert rel_l2(got[k], want[k]) < 1e-12, ("state", k)
